@@ -1,0 +1,96 @@
+/*
+ * CPU oracle for the NEMO-3D sp3d5s* eigensolver hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference algorithm, used by tests/, by the smoke check and by the
+ * cpu_baseline leg of bench.py as the checker.  The product path
+ * (nemo3d_amd/ and libnemohip.so) never links, loads or calls it.
+ *
+ * Parity status per function (how each one is pinned):
+ *   or_sk_bond      - golden GaAs bulk 20-band spectrum (40 k-points,
+ *                     nemo3d/example/TestRun/GaAs_bulk.old_nd_Ek_ascii),
+ *                     the silicon4 Rayleigh quotients, and for strained
+ *                     bonds the reference's own h3d_offdiag_diag_corr
+ *                     compiled into oracle/_ref (when it builds).
+ *   or_onsite       - same goldens; and the reference h3d_diag via _ref.
+ *   or_assemble     - silicon4 (64,000-atom Si box, dangling-bond
+ *                     passivation) Rayleigh quotients and residuals of the
+ *                     shipped eigenvectors; GaAs bulk golden.
+ *   or_hv           - dense-matrix brute force on small boxes.
+ *   or_lanczos_*    - closed forms (2-D Laplacian spectrum), dense eigh.
+ */
+#ifndef NEMO_ORACLE_H
+#define NEMO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of stored onsite values per atom: 10 spatial diagonal energies,
+ * 6 surface s-p couplings (1,2),(1,3),(1,4),(2,3),(2,4),(3,4) and the
+ * spin-orbit constant lambda = Delta/3. */
+#define OR_ONSITE_STRIDE 17
+
+typedef struct or_model {
+  int nb;                /* 10 (spinless sp3d5s*) or 20 (with spin) */
+  int n_atoms;
+  int n_species;
+  const double *parmat;  /* [S][S][35], indexed [cation][anion] */
+  const double *strnmat; /* [S][S][33], indexed [cation][anion] */
+  const int *species;    /* [A] species index into parmat */
+  const int *is_cation;  /* [A] 1 = cation sublattice role */
+  const int *nbr;        /* [A][4] neighbour atom, -1 = dangling */
+  const int *bond_q;     /* [A][4][3] ideal this->nbr vector, units a/4 */
+  const double *disp;    /* [A][3] displacement from ideal position (nm) */
+  double lattice[3];     /* a_x, a_y, a_z (nm) */
+  int passivate;         /* 1 = dangling-bond passivation */
+  double db_shift;       /* DanglingBondEnergyShift (eV) */
+  const double *phi;     /* [A] electrostatic potential (eV) subtracted from
+                            the onsite diagonal, or NULL */
+} or_model;
+
+/* Slater-Koster block of one bond in anion-row orientation:
+ * T[alpha][beta] = <alpha on anion | H | beta on cation>, evaluated with
+ * the strained bond vector nnv (anion -> cation) and the strain-scaled
+ * two-centre integrals; T0 is the same block for the ideal bond nnv0 and
+ * the unscaled integrals.  corr_a/corr_c receive the Boykin diagonal
+ * corrections of the anion/cation orbitals.
+ * par = parmat[cation][anion], strn = strnmat[cation][anion] (may be NULL:
+ * no scaling and no corrections), a0 = unstrained lattice constant. */
+void or_sk_bond(const double *par, const double *strn, double a0,
+                const double nnv[3], const double nnv0[3], double T[100],
+                double corr_a[10], double corr_c[10]);
+
+/* Onsite spatial diagonal (10) and lambda from an averaged parameter set;
+ * anion = 1 selects the anion energies. */
+void or_onsite(const double *param, int anion, double diag[10],
+               double *lambda);
+
+/* Full assembly in the row-owner layout: onsite[A][17] and
+ * blocks[A][4][10][10] with y_i += blocks[i][n] * x_{nbr[i][n]} per spin.
+ * Returns the number of surface atoms. */
+int or_assemble(const or_model *m, double *onsite, double *blocks);
+
+/* y = (H - shift I) x for complex vectors of length A*nb (interleaved
+ * re,im).  phase (optional, [A][4][2]) multiplies each neighbour block by
+ * a Bloch factor. */
+void or_hv(int nb, int n_atoms, const double *onsite, const double *blocks,
+           const int *nbr, const double *phase, double shift, const double *x,
+           double *y);
+
+/* Reference-style Lanczos (no reorthogonalisation): runs `iters` steps from
+ * the normalised start r0 and returns the tridiagonal alpha[iters],
+ * beta[iters] (beta[k] couples steps k and k+1). */
+void or_lanczos_tridiag(int nb, int n_atoms, const double *onsite,
+                        const double *blocks, const int *nbr, const double *r0,
+                        int iters, double *alpha, double *beta);
+
+/* Eigenvalues of the symmetric tridiagonal (d, e) inside (lo, hi] by
+ * Sturm-sequence bisection; returns the count written to w. */
+int or_tridiag_eigs(int n, const double *d, const double *e, double lo,
+                    double hi, double tol, double *w, int wmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+REFERENCE = "/root/reference/NEMO_3D"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP) device")
+
+
+@pytest.fixture(scope="session")
+def reference_root():
+    if not os.path.isdir(REFERENCE):
+        pytest.skip("reference tree not present")
+    return REFERENCE
