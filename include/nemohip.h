@@ -1,0 +1,170 @@
+/*
+ * libnemohip: MI355X (gfx950) implementation of the NEMO-3D sp3d5s*
+ * eigensolver hot path behind a plain C ABI.
+ *
+ * Conventions (same as the reference operator boundary):
+ *   - complex numbers are interleaved {double re, im} (nml_dcscalar /
+ *     `complex`, NEMO_3D/nml/include, layout-compatible with nh_cplx);
+ *   - vectors are in Hamiltonian row order: atom-major, 10 (spinless) or 20
+ *     (spin up orbitals 0-9 then spin down 10-19) rows per atom;
+ *   - the operator is y = (H - shift I) x and y is fully overwritten
+ *     (Hmult_spds_complete, NEMO_3D/nemo3d/src/base/h_cvectr_mult.c:262-667).
+ * Every entry point returns an nh_status instead of aborting (the
+ * reference calls die()); nh_last_error() gives the message.
+ */
+#ifndef NEMOHIP_H
+#define NEMOHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nh_cplx {
+  double re, im;
+} nh_cplx;
+
+typedef enum nh_status {
+  NH_OK = 0,
+  NH_ERR_ARG = 1,     /* invalid argument or shape */
+  NH_ERR_HIP = 2,     /* HIP runtime error */
+  NH_ERR_STATE = 3,   /* call out of order (e.g. H·v before a build) */
+  NH_ERR_NOMEM = 4,   /* device allocation failed */
+  NH_ERR_NOCONV = 5   /* solver stopped before convergence */
+} nh_status;
+
+/* Band models (Dev.band_model; basis_init, ham3d_final.c:576-620). */
+enum { NH_BANDS_10_SP3D5SS_NOSPIN = 10, NH_BANDS_20_SP3D5SS_SPIN = 20 };
+
+/* Storage layouts of the assembled Hamiltonian. */
+enum {
+  NH_LAYOUT_BLOCKS = 0, /* explicit 10x10 real block per (atom, neighbour) */
+  NH_LAYOUT_SK = 1,     /* per-bond direction cosines + scaled integrals */
+  NH_LAYOUT_MATRIX_FREE = 2 /* bond geometry only; SK + strain every call */
+};
+/* Arithmetic of the stored values and of the per-block products. */
+enum { NH_PREC_FP64 = 0, NH_PREC_FP32 = 1 };
+/* Where vector arguments live. */
+enum { NH_PTR_HOST = 0, NH_PTR_DEVICE = 1 };
+
+/* Model: what setup_hamiltonian (ham3d_final.c:731-1230) hands to the
+ * assembly.  All arrays are host arrays, copied at context creation. */
+typedef struct nh_model {
+  int32_t nb;            /* 10 or 20 */
+  int32_t n_atoms;
+  int32_t n_species;
+  const double *parmat;  /* [S][S][35] indexed [cation][anion] (param_init,
+                            mat3d.c:133-307) */
+  const double *strnmat; /* [S][S][33] indexed [cation][anion] */
+  const int32_t *species;   /* [A] */
+  const int32_t *is_cation; /* [A] 1 on the cation role */
+  const int32_t *nbr;       /* [A][4] neighbour atom, -1 dangling */
+  const int32_t *bond_q;    /* [A][4][3] ideal this->nbr vector, units a/4 */
+  const double *disp;       /* [A][3] displacement from the ideal site, nm */
+  double lattice[3];        /* cubic cell lengths, nm */
+  int32_t passivate;        /* 1: PassivateDanglingBond */
+  double db_shift;          /* DanglingBondEnergyShift, eV */
+  const double *phi;        /* [A] potential subtracted from the diagonal,
+                               or NULL (Phys.PotentialFile) */
+} nh_model;
+
+typedef struct nh_context nh_context;
+
+/* Context: copies the model to the device (setup_hamiltonian's outputs). */
+nh_status nh_context_create(const nh_model *model, int device,
+                            nh_context **out);
+void nh_context_destroy(nh_context *ctx);
+int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
+
+/* Assemble H on the device in the given layout and precision.  Replaces
+ * the first-call assembly of Hmult_spds_col (h_cvectr_mult.c:1134-1886)
+ * with its storage into HamZincBlende (Hmult_spds_col_prep_store,
+ * h_cvectr_mult.c:1888-2082). */
+nh_status nh_ham_build(nh_context *ctx, int layout, int precision);
+
+/* y = (H - shift I) x.  Replaces Hmult_spds_complete
+ * (h_cvectr_mult.c:262-667) / Hmult_spds_col_stored (Hmult_stored.c:73-172). */
+nh_status nh_hv(nh_context *ctx, double shift, const nh_cplx *x, nh_cplx *y,
+                int ptr_kind);
+
+/* Y = (H - shift I) X for ncol column vectors stored with leading
+ * dimension ld (>= N) in elements.  Replaces Hmatmult_spds_complete
+ * (h_cmat_mult.c:85-...) / Hmatmult_spds_col_stored
+ * (Hmult_stored.c:2359-2474). */
+nh_status nh_hV(nh_context *ctx, double shift, const nh_cplx *X, nh_cplx *Y,
+                int ncol, int64_t ld, int ptr_kind);
+
+/* Copy the assembled FP64 Hamiltonian back (row-owner layout): onsite
+ * [A][17] = 10 diagonal energies, 6 surface s-p couplings
+ * (1,2),(1,3),(1,4),(2,3),(2,4),(3,4), spin-orbit lambda; blocks
+ * [A][4][10][10] with y_i += blocks[i][n] x_nbr(i,n) per spin.  For the
+ * parity tests; requires a build in NH_LAYOUT_BLOCKS / NH_PREC_FP64. */
+nh_status nh_ham_export(nh_context *ctx, double *onsite, double *blocks);
+
+/* ---- device buffers -------------------------------------------------------
+ * The library owns its device memory (it uses the system ROCm runtime, so
+ * it does not share allocations with other HIP runtimes in the process). */
+nh_status nh_device_alloc(nh_context *ctx, int64_t bytes, void **dptr);
+nh_status nh_device_free(nh_context *ctx, void *dptr);
+nh_status nh_memcpy_h2d(nh_context *ctx, void *dst, const void *src,
+                        int64_t bytes);
+nh_status nh_memcpy_d2h(nh_context *ctx, void *dst, const void *src,
+                        int64_t bytes);
+
+/* ---- vector kernels (device pointers, length n complex) ---------------
+ * Reductions accumulate in FP64 (vect_dot_vect, h_cvectr_mult.c:138-154;
+ * Lanczos inner products, lanczos.c:679-754). */
+nh_status nh_dot(nh_context *ctx, int64_t n, const nh_cplx *x,
+                 const nh_cplx *y, nh_cplx *out /* host: sum conj(x) y */);
+nh_status nh_nrm2(nh_context *ctx, int64_t n, const nh_cplx *x,
+                  double *out /* host */);
+/* y = a x + b y (a, b complex) */
+nh_status nh_axpby(nh_context *ctx, int64_t n, nh_cplx a, const nh_cplx *x,
+                   nh_cplx b, nh_cplx *y);
+
+/* ---- solvers ----------------------------------------------------------- */
+
+/* Lanczos options: the Lanczos_struct fields of the reference deck
+ * (XMLtoC/apps/ioNemo3D/src/Lanczos_struct.H:8-30). */
+typedef struct nh_lanczos_opts {
+  int32_t max_iter;             /* MaxIter */
+  double conv_tolerance;        /* ConvTolerance */
+  int32_t num_eig_vb;           /* NumEigReq_vb */
+  int32_t num_eig_cb;           /* NumEigReq_cb */
+  double erange_vb[2];          /* Erange_vb */
+  double erange_cb[2];          /* Erange_cb */
+  int32_t conv_check_start;     /* ConvCheckStartIter */
+  int32_t conv_check_skip;      /* ConvCheckSkipRate */
+  int32_t start;                /* 0 CyclicRandom, 1 Const1, 2 user vector */
+  int32_t cyclic_len;           /* CyclicString length */
+  const double *cyclic_string;  /* CyclicString values */
+  const nh_cplx *start_vector;  /* start == 2: host vector of length N */
+  double resid_tol;             /* relative residual target of the final
+                                   FP64 refinement (0: no refinement) */
+  int32_t want_vectors;         /* compute eigenvectors (second pass) */
+} nh_lanczos_opts;
+
+typedef struct nh_eig_result {
+  int32_t n_found;       /* converged pairs written */
+  int32_t iterations;    /* Lanczos iterations of the eigenvalue pass */
+  int32_t matvecs;       /* H·v products, all passes */
+  double *eigenvalues;   /* caller array, capacity `capacity` */
+  double *residuals;     /* caller array: ||H psi - E psi|| / ||psi|| */
+  nh_cplx *eigenvectors; /* caller host array capacity*N, or NULL */
+  int32_t capacity;
+  double seconds;        /* wall time of the solve */
+} nh_eig_result;
+
+/* Lanczos without reorthogonalisation, eigenvalues from the tridiagonal
+ * in the vb/cb windows, eigenvectors by a second pass
+ * (eigensolve_lanczos, NEMO_3D/eigen/src/lanczos.c:1823-2089). */
+nh_status nh_eigensolve_lanczos(nh_context *ctx, const nh_lanczos_opts *opts,
+                                nh_eig_result *res);
+
+const char *nh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

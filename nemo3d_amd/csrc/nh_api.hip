@@ -1,0 +1,278 @@
+// C-ABI entry points of libnemohip (declared in include/nemohip.h).
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "nh_internal.h"
+
+namespace nh {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+nh_status ensure_buffers(nh_context *c, int64_t elems) {
+  if (elems <= c->buf_cap) return NH_OK;
+  hipFree(c->xbuf);
+  hipFree(c->ybuf);
+  c->xbuf = c->ybuf = nullptr;
+  c->buf_cap = 0;
+  NH_HIP(hipMalloc(&c->xbuf, elems * sizeof(nh_cplx)));
+  NH_HIP(hipMalloc(&c->ybuf, elems * sizeof(nh_cplx)));
+  c->buf_cap = elems;
+  return NH_OK;
+}
+
+template <typename T>
+static nh_status upload(T **dst, const T *src, size_t count) {
+  NH_HIP(hipMalloc(dst, count * sizeof(T) + 16));
+  NH_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return NH_OK;
+}
+
+static void destroy(nh_context *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  void *ptrs[] = {c->parmat, c->strnmat, c->species, c->is_cation, c->nbr,
+                  c->bond_q, c->disp, c->phi, c->cations, c->bond_of,
+                  c->rev_slot, c->onsite, c->blocks64, c->blocks32,
+                  c->onsite32, c->sk64, c->sk32, c->corr, c->red, c->xbuf,
+                  c->ybuf};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->hred) hipHostFree(c->hred);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static nh_status create(const nh_model *m, int device, nh_context **out) {
+  NH_CHECK(m && out, NH_ERR_ARG, "null model or output pointer");
+  NH_CHECK(m->nb == 10 || m->nb == 20, NH_ERR_ARG, "nb must be 10 or 20");
+  NH_CHECK(m->n_atoms > 0 && m->n_species > 0, NH_ERR_ARG, "empty model");
+  NH_CHECK(m->parmat && m->strnmat && m->species && m->is_cation && m->nbr &&
+               m->bond_q && m->disp,
+           NH_ERR_ARG, "missing model array");
+  const int A = m->n_atoms, S = m->n_species;
+  NH_CHECK((int64_t)A * m->nb < ((int64_t)1 << 40), NH_ERR_ARG, "too large");
+  std::vector<int32_t> cations, bond_of((size_t)A * 4, -1),
+      rev((size_t)A * 4, -1);
+  for (int i = 0; i < A; ++i) {
+    NH_CHECK(m->species[i] >= 0 && m->species[i] < S, NH_ERR_ARG,
+             "species[%d] out of range", i);
+    int present = 0;
+    for (int n = 0; n < 4; ++n) {
+      const int j = m->nbr[i * 4 + n];
+      NH_CHECK(j >= -1 && j < A && j != i, NH_ERR_ARG, "nbr[%d][%d] invalid",
+               i, n);
+      if (j < 0) continue;
+      ++present;
+      NH_CHECK((m->is_cation[i] != 0) != (m->is_cation[j] != 0), NH_ERR_ARG,
+               "bond %d-%d does not join a cation and an anion", i, j);
+      int r = -1;
+      for (int q = 0; q < 4; ++q)
+        if (m->nbr[j * 4 + q] == i) r = q;
+      NH_CHECK(r >= 0, NH_ERR_ARG, "neighbour table not symmetric at %d-%d",
+               i, j);
+      rev[i * 4 + n] = r;
+    }
+    NH_CHECK(present > 0, NH_ERR_ARG, "atom %d has no neighbours", i);
+  }
+  for (int i = 0; i < A; ++i)
+    if (m->is_cation[i]) cations.push_back(i);
+  for (size_t ci = 0; ci < cations.size(); ++ci) {
+    const int i = cations[ci];
+    for (int n = 0; n < 4; ++n) {
+      const int j = m->nbr[i * 4 + n];
+      if (j < 0) continue;
+      bond_of[i * 4 + n] = (int32_t)(ci * 4 + n);
+      bond_of[j * 4 + rev[i * 4 + n]] = (int32_t)(ci * 4 + n);
+    }
+  }
+  NH_CHECK(!cations.empty(), NH_ERR_ARG, "no cation-role atoms");
+
+  NH_HIP(hipSetDevice(device));
+  nh_context *c = new nh_context();
+  c->device = device;
+  c->nb = m->nb;
+  c->n_atoms = A;
+  c->n_species = S;
+  c->n_cations = (int)cations.size();
+  c->n = (int64_t)A * m->nb;
+  for (int k = 0; k < 3; ++k) c->lattice[k] = m->lattice[k];
+  c->passivate = m->passivate;
+  c->db_shift = m->db_shift;
+  nh_status st = NH_OK;
+#define UP(dst, src, cnt)                              \
+  if (st == NH_OK) st = upload(&c->dst, src, (size_t)(cnt));
+  UP(parmat, m->parmat, (size_t)S * S * 35);
+  UP(strnmat, m->strnmat, (size_t)S * S * 33);
+  UP(species, m->species, A);
+  UP(is_cation, m->is_cation, A);
+  UP(nbr, m->nbr, (size_t)A * 4);
+  UP(bond_q, m->bond_q, (size_t)A * 12);
+  UP(disp, m->disp, (size_t)A * 3);
+  if (m->phi) UP(phi, m->phi, A);
+  UP(cations, cations.data(), cations.size());
+  UP(bond_of, bond_of.data(), (size_t)A * 4);
+  UP(rev_slot, rev.data(), (size_t)A * 4);
+#undef UP
+  if (st == NH_OK) {
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+      set_error("hipStreamCreate: %s", hipGetErrorString(e));
+      st = NH_ERR_HIP;
+    }
+  }
+  if (st != NH_OK) {
+    destroy(c);
+    return st;
+  }
+  *out = c;
+  return NH_OK;
+}
+
+static nh_status check_ctx(nh_context *c) {
+  NH_CHECK(c, NH_ERR_ARG, "null context");
+  (void)hipGetLastError();  // errors of earlier, already reported calls
+  NH_HIP(hipSetDevice(c->device));
+  return NH_OK;
+}
+
+}  // namespace nh
+
+using namespace nh;
+
+extern "C" {
+
+const char *nh_last_error(void) { return g_err; }
+
+nh_status nh_context_create(const nh_model *model, int device,
+                            nh_context **out) {
+  return create(model, device, out);
+}
+
+void nh_context_destroy(nh_context *ctx) { destroy(ctx); }
+
+int64_t nh_context_dim(const nh_context *ctx) { return ctx ? ctx->n : 0; }
+
+nh_status nh_ham_build(nh_context *ctx, int layout, int precision) {
+  NH_TRY(check_ctx(ctx));
+  return assemble(ctx, layout, precision);
+}
+
+nh_status nh_hv(nh_context *ctx, double shift, const nh_cplx *x, nh_cplx *y,
+                int ptr_kind) {
+  return nh_hV(ctx, shift, x, y, 1, ctx ? ctx->n : 0, ptr_kind);
+}
+
+nh_status nh_hV(nh_context *ctx, double shift, const nh_cplx *X, nh_cplx *Y,
+                int ncol, int64_t ld, int ptr_kind) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(X && Y && X != Y, NH_ERR_ARG, "H·v: x and y must be distinct");
+  NH_CHECK(ncol >= 1 && ld >= ctx->n, NH_ERR_ARG, "H·V: bad ncol/ld");
+  if (ptr_kind == NH_PTR_DEVICE) {
+    NH_TRY(launch_hv(ctx, shift, X, Y, ncol, ld));
+    NH_HIP(hipStreamSynchronize(ctx->stream));
+    return NH_OK;
+  }
+  NH_CHECK(ptr_kind == NH_PTR_HOST, NH_ERR_ARG, "bad ptr_kind");
+  const int64_t n = ctx->n;
+  NH_TRY(ensure_buffers(ctx, n * ncol));
+  for (int k = 0; k < ncol; ++k)
+    NH_HIP(hipMemcpyAsync(ctx->xbuf + k * n, X + k * ld, n * sizeof(nh_cplx),
+                          hipMemcpyHostToDevice, ctx->stream));
+  NH_TRY(launch_hv(ctx, shift, ctx->xbuf, ctx->ybuf, ncol, n));
+  for (int k = 0; k < ncol; ++k)
+    NH_HIP(hipMemcpyAsync(Y + k * ld, ctx->ybuf + k * n, n * sizeof(nh_cplx),
+                          hipMemcpyDeviceToHost, ctx->stream));
+  NH_HIP(hipStreamSynchronize(ctx->stream));
+  return NH_OK;
+}
+
+nh_status nh_ham_export(nh_context *ctx, double *onsite, double *blocks) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(ctx->onsite && ctx->blocks64, NH_ERR_STATE, "not built");
+  const int A = ctx->n_atoms;
+  if (onsite) {
+    std::vector<double> tmp((size_t)A * kOnsiteStride);
+    NH_HIP(hipMemcpy(tmp.data(), ctx->onsite, tmp.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int i = 0; i < A; ++i)
+      memcpy(onsite + (size_t)i * 17, tmp.data() + (size_t)i * kOnsiteStride,
+             17 * sizeof(double));
+  }
+  if (blocks)
+    NH_HIP(hipMemcpy(blocks, ctx->blocks64, (size_t)A * 400 * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return NH_OK;
+}
+
+nh_status nh_device_alloc(nh_context *ctx, int64_t bytes, void **dptr) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(dptr && bytes >= 0, NH_ERR_ARG, "nh_device_alloc: bad arguments");
+  hipError_t e = hipMalloc(dptr, bytes > 0 ? bytes : 16);
+  NH_CHECK(e == hipSuccess, NH_ERR_NOMEM, "hipMalloc(%lld): %s",
+           (long long)bytes, hipGetErrorString(e));
+  return NH_OK;
+}
+
+nh_status nh_device_free(nh_context *ctx, void *dptr) {
+  NH_TRY(check_ctx(ctx));
+  NH_HIP(hipStreamSynchronize(ctx->stream));
+  if (dptr) NH_HIP(hipFree(dptr));
+  return NH_OK;
+}
+
+nh_status nh_memcpy_h2d(nh_context *ctx, void *dst, const void *src,
+                        int64_t bytes) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(dst && src && bytes >= 0, NH_ERR_ARG, "nh_memcpy_h2d: bad arguments");
+  NH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  NH_HIP(hipStreamSynchronize(ctx->stream));
+  return NH_OK;
+}
+
+nh_status nh_memcpy_d2h(nh_context *ctx, void *dst, const void *src,
+                        int64_t bytes) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(dst && src && bytes >= 0, NH_ERR_ARG, "nh_memcpy_d2h: bad arguments");
+  NH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  NH_HIP(hipStreamSynchronize(ctx->stream));
+  return NH_OK;
+}
+
+nh_status nh_dot(nh_context *ctx, int64_t n, const nh_cplx *x,
+                 const nh_cplx *y, nh_cplx *out) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(x && y && out && n >= 0, NH_ERR_ARG, "nh_dot: bad arguments");
+  double r[2];
+  NH_TRY(reduce_dot(ctx, n, x, y, r));
+  out->re = r[0];
+  out->im = r[1];
+  return NH_OK;
+}
+
+nh_status nh_nrm2(nh_context *ctx, int64_t n, const nh_cplx *x, double *out) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(x && out && n >= 0, NH_ERR_ARG, "nh_nrm2: bad arguments");
+  double s;
+  NH_TRY(reduce_nrm2sq(ctx, n, x, &s));
+  *out = sqrt(s);
+  return NH_OK;
+}
+
+nh_status nh_axpby(nh_context *ctx, int64_t n, nh_cplx a, const nh_cplx *x,
+                   nh_cplx b, nh_cplx *y) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(x && y && n >= 0, NH_ERR_ARG, "nh_axpby: bad arguments");
+  NH_TRY(launch_axpby(ctx, n, a, x, b, y));
+  NH_HIP(hipStreamSynchronize(ctx->stream));
+  return NH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Sparse H·v and block H·V for the stored-block layout (row computation
+// in nh_hv_row.h).
+#include "nh_hv_row.h"
+
+namespace nh {
+namespace {
+
+template <int NSP, typename TB, typename TA>
+__global__ __launch_bounds__(kThreads) void k_hv_blocks(
+    int A, const double *__restrict__ onsite, const TB *__restrict__ blocks,
+    const int32_t *__restrict__ nbr, double shift,
+    const double2 *__restrict__ x, double2 *__restrict__ y, int64_t ld) {
+  constexpr int NB = NSP * 10;
+  const int atom = blockIdx.x * kAtomsPerBlock + threadIdx.x / 10;
+  const int o = threadIdx.x % 10;
+  if (atom >= A) return;
+  const double2 *xc = x + (int64_t)blockIdx.y * ld;
+  double2 *yc = y + (int64_t)blockIdx.y * ld;
+  double acc_re[NSP], acc_im[NSP];
+  double2 xi[NSP];
+  hv_row<NSP, TB, TA>(atom, o, onsite, blocks, nbr, shift, xc, acc_re, acc_im,
+                      xi);
+  const int64_t base = (int64_t)atom * NB;
+#pragma unroll
+  for (int s = 0; s < NSP; ++s)
+    yc[base + s * 10 + o] = make_double2(acc_re[s], acc_im[s]);
+}
+
+}  // namespace
+
+nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
+                    int ncol, int64_t ld) {
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS, NH_ERR_STATE,
+           "H·v: Hamiltonian not built in a supported layout");
+  NH_CHECK(ncol >= 1 && ncol <= 65535 && ld >= c->n, NH_ERR_ARG,
+           "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
+  const dim3 grid((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock, ncol);
+  const double2 *xd = reinterpret_cast<const double2 *>(x);
+  double2 *yd = reinterpret_cast<double2 *>(y);
+  if (c->precision == NH_PREC_FP64) {
+    if (c->nb == 20)
+      hipLaunchKernelGGL((k_hv_blocks<2, double, double>), grid, dim3(kThreads), 0,
+                         c->stream, c->n_atoms, c->onsite, c->blocks64, c->nbr,
+                         shift, xd, yd, ld);
+    else
+      hipLaunchKernelGGL((k_hv_blocks<1, double, double>), grid, dim3(kThreads), 0,
+                         c->stream, c->n_atoms, c->onsite, c->blocks64, c->nbr,
+                         shift, xd, yd, ld);
+  } else {
+    if (c->nb == 20)
+      hipLaunchKernelGGL((k_hv_blocks<2, float, float>), grid, dim3(kThreads), 0,
+                         c->stream, c->n_atoms, c->onsite, c->blocks32, c->nbr,
+                         shift, xd, yd, ld);
+    else
+      hipLaunchKernelGGL((k_hv_blocks<1, float, float>), grid, dim3(kThreads), 0,
+                         c->stream, c->n_atoms, c->onsite, c->blocks32, c->nbr,
+                         shift, xd, yd, ld);
+  }
+  NH_HIP(hipGetLastError());
+  return NH_OK;
+}
+
+}  // namespace nh

@@ -1,0 +1,106 @@
+// Internal state of libnemohip (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/nemohip.h"
+
+namespace nh {
+
+void set_error(const char *fmt, ...);
+
+#define NH_HIP(call)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      ::nh::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,          \
+                      hipGetErrorString(e_));                              \
+      return NH_ERR_HIP;                                                   \
+    }                                                                      \
+  } while (0)
+
+#define NH_CHECK(cond, code, ...)                                          \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      ::nh::set_error(__VA_ARGS__);                                        \
+      return code;                                                         \
+    }                                                                      \
+  } while (0)
+
+#define NH_TRY(expr)                                                       \
+  do {                                                                     \
+    nh_status s_ = (expr);                                                 \
+    if (s_ != NH_OK) return s_;                                            \
+  } while (0)
+
+// Onsite record per atom: 10 diagonal energies, 6 surface s-p couplings,
+// spin-orbit lambda (17 doubles, padded to 18 for 16-byte alignment).
+constexpr int kOnsiteStride = 18;
+// Per-bond SK record: l, m, n and the 21 scaled integrals (24 values).
+constexpr int kSkStride = 24;
+
+}  // namespace nh
+
+struct nh_context {
+  int device = 0;
+  int nb = 0;        // 10 or 20
+  int n_atoms = 0;
+  int n_species = 0;
+  int n_cations = 0;
+  int64_t n = 0;     // n_atoms * nb
+  double lattice[3] = {0, 0, 0};
+  int passivate = 0;
+  double db_shift = 0;
+  hipStream_t stream = nullptr;
+
+  // model (device)
+  double *parmat = nullptr;     // [S][S][35]
+  double *strnmat = nullptr;    // [S][S][33]
+  int32_t *species = nullptr;   // [A]
+  int32_t *is_cation = nullptr; // [A]
+  int32_t *nbr = nullptr;       // [A][4]
+  int32_t *bond_q = nullptr;    // [A][4][3]
+  double *disp = nullptr;       // [A][3]
+  double *phi = nullptr;        // [A] or null
+  int32_t *cations = nullptr;   // [Nc] atom index of each cation
+  int32_t *bond_of = nullptr;   // [A][4] cation-owned bond id (ci*4+slot)
+  int32_t *rev_slot = nullptr;  // [A][4] slot of this atom in the nbr's list
+
+  // assembled Hamiltonian
+  int layout = -1;
+  int precision = -1;
+  double *onsite = nullptr;     // [A][kOnsiteStride]
+  double *blocks64 = nullptr;   // [A][4][100] (NH_LAYOUT_BLOCKS, FP64)
+  float *blocks32 = nullptr;    // [A][4][100] (NH_LAYOUT_BLOCKS, FP32)
+  float *onsite32 = nullptr;    // [A][kOnsiteStride] (FP32 runs)
+  double *sk64 = nullptr;       // [Nc*4][kSkStride] (NH_LAYOUT_SK)
+  float *sk32 = nullptr;
+  double *corr = nullptr;       // [A][4][10] Boykin corrections (scratch)
+
+  // work buffers
+  double *red = nullptr;        // reduction partials
+  int red_cap = 0;
+  double *hred = nullptr;       // pinned host mirror of reductions
+  nh_cplx *xbuf = nullptr;      // host-pointer staging
+  nh_cplx *ybuf = nullptr;
+  int64_t buf_cap = 0;          // complex elements in xbuf/ybuf
+};
+
+namespace nh {
+// kernels / launchers (nh_assemble.hip, nh_hv.hip, nh_vector.hip)
+nh_status assemble(nh_context *c, int layout, int precision);
+nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x,
+                    nh_cplx *y, int ncol, int64_t ld);
+nh_status reduce_dot(nh_context *c, int64_t n, const nh_cplx *x,
+                     const nh_cplx *y, double *out2 /* host re,im */);
+nh_status reduce_nrm2sq(nh_context *c, int64_t n, const nh_cplx *x,
+                        double *out);
+nh_status launch_axpby(nh_context *c, int64_t n, nh_cplx a, const nh_cplx *x,
+                       nh_cplx b, nh_cplx *y);
+nh_status ensure_reduce(nh_context *c, int blocks);
+nh_status ensure_buffers(nh_context *c, int64_t elems);
+}  // namespace nh

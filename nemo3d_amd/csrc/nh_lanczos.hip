@@ -1,0 +1,532 @@
+// Device-resident Lanczos eigensolver (no reorthogonalisation), the
+// counterpart of eigensolve_lanczos (NEMO_3D/eigen/src/lanczos.c:1823-2089).
+//
+// One iteration (single_lanczos_iteration, lanczos.c:421-757) is two fused
+// kernels plus two one-workgroup reductions, all on one stream:
+//   k_lz_hv     q = r / beta_{k-1} (written into a ring slot), y = H q, and
+//               per-workgroup partials of alpha = Re<q, y>;
+//   k_lz_update r = y - alpha q - beta_{k-1} q_prev with partials of |r|^2.
+// alpha and beta stay in device arrays; the host copies the tridiagonal
+// every ConvCheckSkipRate iterations and analyses it (Sturm bisection in
+// the Erange windows, convergence by the trailing components of the
+// tridiagonal eigenvector, find_converged_eigval Method 2,
+// lanczos.c:764-996).  Eigenvectors come from a second pass over the same
+// recurrence (eigenvectorsLanczos, lanczos.c:1364-1689): the Lanczos
+// vectors land in a ring of kRing slots and every kRing iterations one
+// kernel adds Ring * Z to the eigenvector block.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "nh_hv_row.h"
+#include "nh_vector.h"
+
+namespace nh {
+namespace {
+
+constexpr int kRing = 32;
+constexpr int kMaxVec = 64;
+
+__device__ __forceinline__ double wg_sum320(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __shared__ double s[kThreads / 64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) s[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kThreads / 64; ++k) t += s[k];
+  }
+  return t;
+}
+
+template <int NSP, typename TB, typename TA>
+__global__ __launch_bounds__(kThreads) void k_lz_hv(
+    int A, const double *__restrict__ onsite, const TB *__restrict__ blocks,
+    const int32_t *__restrict__ nbr, const double2 *__restrict__ R,
+    double2 *__restrict__ Q, double2 *__restrict__ Y,
+    const double *__restrict__ beta_prev, double *__restrict__ part) {
+  constexpr int NB = NSP * 10;
+  const int atom = blockIdx.x * kAtomsPerBlock + threadIdx.x / 10;
+  const int o = threadIdx.x % 10;
+  const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
+  double local = 0.0;
+  if (atom < A) {
+    double acc_re[NSP], acc_im[NSP];
+    double2 xi[NSP];
+    hv_row<NSP, TB, TA>(atom, o, onsite, blocks, nbr, 0.0, R, acc_re, acc_im,
+                        xi);
+    const int64_t base = (int64_t)atom * NB;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+      const double2 q = make_double2(xi[s].x * inv, xi[s].y * inv);
+      const double2 y = make_double2(acc_re[s] * inv, acc_im[s] * inv);
+      Q[base + s * 10 + o] = q;
+      Y[base + s * 10 + o] = y;
+      local += q.x * y.x + q.y * y.y;
+    }
+  }
+  if (part) {
+    const double t = wg_sum320(local);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+// y <- y - alpha q - beta_prev q_prev, partial sums of |y|^2.
+__global__ __launch_bounds__(kRedThreads) void k_lz_update(
+    int64_t n, double2 *__restrict__ Y, const double2 *__restrict__ Q,
+    const double2 *__restrict__ Qp, const double *__restrict__ alpha,
+    const double *__restrict__ beta_prev, double *__restrict__ part) {
+  const double al = alpha[0];
+  const double be = beta_prev ? beta_prev[0] : 0.0;
+  double s = 0.0, z = 0.0;
+  for (int64_t k = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * kRedThreads) {
+    double2 y = Y[k];
+    const double2 q = Q[k];
+    if (Qp) {
+      const double2 p = Qp[k];
+      y.x -= al * q.x + be * p.x;
+      y.y -= al * q.y + be * p.y;
+    } else {
+      y.x -= al * q.x;
+      y.y -= al * q.y;
+    }
+    Y[k] = y;
+    s += y.x * y.x + y.y * y.y;
+  }
+  if (part) {
+    block_sum2(s, z);
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+  }
+}
+
+// out[0] = sum of nparts partials (stride `stride`); sqrt if take_sqrt.
+__global__ __launch_bounds__(kRedThreads) void k_lz_scalar(
+    int nparts, int stride, const double *__restrict__ part,
+    double *__restrict__ out, int take_sqrt) {
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += kRedThreads) a += part[k * stride];
+  block_sum2(a, b);
+  if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(a) : a;
+}
+
+// V[e] += sum_k Z[e][k] Ring[k] for e < ne, k < K.
+__global__ __launch_bounds__(256) void k_lz_accum(
+    int64_t n, int ne, int K, const double *__restrict__ Z,
+    const double2 *__restrict__ ring, int64_t ring_ld, double2 *__restrict__ V,
+    int64_t v_ld) {
+  __shared__ double zs[kMaxVec * kRing];
+  for (int t = threadIdx.x; t < ne * K; t += blockDim.x) zs[t] = Z[t];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double2 q[kRing];
+#pragma unroll
+    for (int k = 0; k < kRing; ++k)
+      q[k] = k < K ? ring[(int64_t)k * ring_ld + i] : make_double2(0.0, 0.0);
+    for (int e = 0; e < ne; ++e) {
+      double2 v = V[(int64_t)e * v_ld + i];
+#pragma unroll
+      for (int k = 0; k < kRing; ++k) {
+        const double z = zs[e * K + (k < K ? k : 0)];
+        if (k < K) {
+          v.x += z * q[k].x;
+          v.y += z * q[k].y;
+        }
+      }
+      V[(int64_t)e * v_ld + i] = v;
+    }
+  }
+}
+
+__global__ void k_scale(int64_t n, double2 *__restrict__ x, double s) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    x[k].x *= s;
+    x[k].y *= s;
+  }
+}
+
+// ---------------- host-side tridiagonal analysis ---------------------------
+
+int sturm(const std::vector<double> &d, const std::vector<double> &e, int m,
+          double x) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (q < 0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    if (q == 0.0) q = 1e-300;
+    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+// k-th smallest eigenvalue (0-based) of T_m by bisection on [lo, hi].
+double bisect(const std::vector<double> &d, const std::vector<double> &e,
+              int m, int k, double lo, double hi, double abstol) {
+  for (int it = 0; it < 200 && hi - lo > abstol; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm(d, e, m, mid) > k) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Eigenvector of T_m at lambda by inverse iteration (tridiagonal Gaussian
+// elimination with partial pivoting), from a fixed pseudo-random start.
+void tri_eigvec(const std::vector<double> &d, const std::vector<double> &e,
+                int m, double lambda, std::vector<double> &x) {
+  x.assign(m, 0.0);
+  uint64_t s = 0x9E3779B97F4A7C15ull;
+  for (int i = 0; i < m; ++i) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
+  }
+  const double pert = 1e-13 * (std::fabs(lambda) + 1.0);
+  std::vector<double> a(m), b(m), c(m), c2(m), rhs(m);
+  std::vector<char> swapped(m);
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    // (T - lambda') x_new = x with a tridiagonal LU (a: sub, b: diag, c: sup)
+    for (int i = 0; i < m; ++i) {
+      b[i] = d[i] - lambda - pert;
+      a[i] = i > 0 ? e[i - 1] : 0.0;
+      c[i] = i + 1 < m ? e[i] : 0.0;
+      c2[i] = 0.0;
+      rhs[i] = x[i];
+    }
+    for (int i = 0; i + 1 < m; ++i) {
+      // rows i and i+1; eliminate a[i+1]
+      if (std::fabs(a[i + 1]) > std::fabs(b[i])) {
+        std::swap(b[i], a[i + 1]);
+        std::swap(c[i], b[i + 1]);
+        std::swap(c2[i], c[i + 1]);
+        std::swap(rhs[i], rhs[i + 1]);
+      }
+      if (b[i] == 0.0) b[i] = 1e-300;
+      const double f = a[i + 1] / b[i];
+      b[i + 1] -= f * c[i];
+      c[i + 1] -= f * c2[i];
+      rhs[i + 1] -= f * rhs[i];
+    }
+    if (b[m - 1] == 0.0) b[m - 1] = 1e-300;
+    x[m - 1] = rhs[m - 1] / b[m - 1];
+    if (m > 1) x[m - 2] = (rhs[m - 2] - c[m - 2] * x[m - 1]) / b[m - 2];
+    for (int i = m - 3; i >= 0; --i)
+      x[i] = (rhs[i] - c[i] * x[i + 1] - c2[i] * x[i + 2]) / b[i];
+    double nrm = 0.0;
+    for (int i = 0; i < m; ++i) nrm += x[i] * x[i];
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < m; ++i) x[i] /= nrm;
+  }
+}
+
+struct Converged {
+  double value;
+  int iter;
+  int window;
+};
+
+class LanczosRun {
+ public:
+  LanczosRun(nh_context *c) : c_(c) {}
+  ~LanczosRun() {
+    for (void *p : allocs_) hipFree(p);
+  }
+  nh_status run(const nh_lanczos_opts *o, nh_eig_result *res);
+
+ private:
+  nh_status alloc(void **p, size_t bytes) {
+    NH_HIP(hipMalloc(p, bytes));
+    allocs_.push_back(*p);
+    return NH_OK;
+  }
+  nh_status iterate(int k, bool with_reductions);
+  nh_status analyse(int m, const nh_lanczos_opts *o);
+  bool window_done(int w, int req) const;
+
+  nh_context *c_;
+  std::vector<void *> allocs_;
+  int64_t n_ = 0;
+  double2 *R_ = nullptr, *Y_ = nullptr, *ring_ = nullptr;
+  double *alpha_ = nullptr, *beta_ = nullptr, *part_ = nullptr;
+  int nparts_ = 0;
+  std::vector<double> ha_, hb_;
+  std::vector<Converged> conv_;
+  std::vector<double> lastRitz_[2];
+  std::vector<char> lastConv_[2];
+};
+
+nh_status LanczosRun::iterate(int k, bool red) {
+  nh_context *c = c_;
+  const int gridA = (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
+  double2 *Q = ring_ + (int64_t)(k % kRing) * n_;
+  double2 *Qp = k > 0 ? ring_ + (int64_t)((k - 1) % kRing) * n_ : nullptr;
+  const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
+  double *part = red ? part_ : nullptr;
+#define LZ_HV(NSP, TB, BLK)                                                   \
+  hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB>), dim3(gridA), dim3(kThreads), 0,  \
+                     c->stream, c->n_atoms, c->onsite, BLK, c->nbr, R_, Q, Y_, \
+                     bprev, part)
+  if (c->precision == NH_PREC_FP64) {
+    if (c->nb == 20) LZ_HV(2, double, c->blocks64);
+    else LZ_HV(1, double, c->blocks64);
+  } else {
+    if (c->nb == 20) LZ_HV(2, float, c->blocks32);
+    else LZ_HV(1, float, c->blocks32);
+  }
+#undef LZ_HV
+  if (red)
+    hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
+                       gridA, 1, part_, alpha_ + k, 0);
+  const int nb = reduce_blocks(n_);
+  hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
+                     n_, Y_, Q, Qp, alpha_ + k, bprev, part);
+  if (red)
+    hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
+                       nb, 2, part_, beta_ + k, 1);
+  NH_HIP(hipGetLastError());
+  std::swap(R_, Y_);  // new r is in Y; old r buffer becomes next y
+  return NH_OK;
+}
+
+bool LanczosRun::window_done(int w, int req) const {
+  if (req <= 0) return true;
+  // the `req` Ritz values nearest the gap (top of the vb window, bottom of
+  // the cb window) must all be converged
+  const auto &r = lastRitz_[w];
+  const auto &cv = lastConv_[w];
+  if ((int)r.size() < req) return false;
+  for (int t = 0; t < req; ++t) {
+    const int idx = w == 0 ? (int)r.size() - 1 - t : t;
+    if (!cv[idx]) return false;
+  }
+  return true;
+}
+
+nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
+  // copy the new part of the tridiagonal
+  const int have = (int)ha_.size();
+  ha_.resize(m);
+  hb_.resize(m);
+  NH_HIP(hipMemcpyAsync(ha_.data() + have, alpha_ + have,
+                        (m - have) * sizeof(double), hipMemcpyDeviceToHost,
+                        c_->stream));
+  NH_HIP(hipMemcpyAsync(hb_.data() + have, beta_ + have,
+                        (m - have) * sizeof(double), hipMemcpyDeviceToHost,
+                        c_->stream));
+  NH_HIP(hipStreamSynchronize(c_->stream));
+  const double tol = o->conv_tolerance;
+  const double resolution = 100.0 * tol;  // lanczos.c:1970
+  const double bias = 1e-80;
+  std::vector<double> x;
+  const int req[2] = {o->num_eig_vb, o->num_eig_cb};
+  const double *win[2] = {o->erange_vb, o->erange_cb};
+  for (int w = 0; w < 2; ++w) {
+    lastRitz_[w].clear();
+    lastConv_[w].clear();
+    if (req[w] <= 0) continue;
+    const double lo = win[w][0], hi = win[w][1];
+    const int klo = sturm(ha_, hb_, m, lo), khi = sturm(ha_, hb_, m, hi);
+    // candidates nearest the gap: top of vb, bottom of cb
+    const int want = req[w] + 24;
+    int k0 = klo, k1 = khi;
+    if (w == 0) k0 = std::max(klo, khi - want);
+    else k1 = std::min(khi, klo + want);
+    std::vector<double> ev;
+    for (int k = k0; k < k1; ++k)
+      ev.push_back(bisect(ha_, hb_, m, k, lo, hi, 1e-13));
+    for (size_t i = 0; i < ev.size(); ++i) {
+      // copies of one eigenvalue (ghosts, degeneracies) count once
+      if (i > 0 && std::fabs(ev[i - 1] - ev[i]) <
+                       std::fabs(bias + ev[i]) * resolution)
+        continue;
+      bool known = false;
+      for (const auto &cv : conv_)
+        if (std::fabs(cv.value - ev[i]) < resolution) known = true;
+      bool ok = known;
+      if (!known) {
+        tri_eigvec(ha_, hb_, m, ev[i], x);
+        ok = std::fabs(x[m - 1]) <= tol && (m < 2 || std::fabs(x[m - 2]) <= tol);
+        if (ok) conv_.push_back({ev[i], m, w});
+      }
+      lastRitz_[w].push_back(ev[i]);
+      lastConv_[w].push_back(ok);
+    }
+  }
+  return NH_OK;
+}
+
+nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
+  nh_context *c = c_;
+  const auto t0 = std::chrono::steady_clock::now();
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS, NH_ERR_STATE,
+           "Lanczos: build the Hamiltonian first");
+  NH_CHECK(o->max_iter > 2 && o->conv_tolerance > 0, NH_ERR_ARG,
+           "Lanczos: bad max_iter / tolerance");
+  n_ = c->n;
+  NH_TRY(alloc((void **)&R_, n_ * sizeof(double2)));
+  NH_TRY(alloc((void **)&Y_, n_ * sizeof(double2)));
+  NH_TRY(alloc((void **)&ring_, (size_t)kRing * n_ * sizeof(double2)));
+  NH_TRY(alloc((void **)&alpha_, (size_t)(o->max_iter + 1) * sizeof(double)));
+  NH_TRY(alloc((void **)&beta_, (size_t)(o->max_iter + 1) * sizeof(double)));
+  nparts_ = std::max((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock,
+                     2 * kRedMaxBlocks) + 64;
+  NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * sizeof(double)));
+
+  // start vector (lanczos_initialize, eigsys3d_par.c:113-215)
+  std::vector<nh_cplx> r0(n_);
+  if (o->start == 0) {
+    NH_CHECK(o->cyclic_len > 0 && o->cyclic_string, NH_ERR_ARG,
+             "CyclicRandom start needs CyclicString");
+    const int L = o->cyclic_len;
+    for (int64_t j = 0; j < n_; ++j) {
+      r0[j].re = o->cyclic_string[(2 * j) % L];
+      r0[j].im = o->cyclic_string[(2 * j + 1) % L];
+    }
+  } else if (o->start == 1) {
+    for (int64_t j = 0; j < n_; ++j) r0[j].re = r0[j].im = 1.0;
+  } else {
+    NH_CHECK(o->start_vector, NH_ERR_ARG, "start vector missing");
+    for (int64_t j = 0; j < n_; ++j) r0[j] = o->start_vector[j];
+  }
+  double s = 0.0;
+  for (int64_t j = 0; j < n_; ++j) s += r0[j].re * r0[j].re + r0[j].im * r0[j].im;
+  NH_CHECK(s > 0, NH_ERR_ARG, "zero start vector");
+  s = std::sqrt(s);
+  for (int64_t j = 0; j < n_; ++j) {
+    r0[j].re /= s;
+    r0[j].im /= s;
+  }
+  NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
+                        hipMemcpyHostToDevice, c->stream));
+
+  // pass 1: eigenvalues
+  int m = 0, skip = 0;
+  bool done = false;
+  int matvecs = 0;
+  while (!done && m + 1 < o->max_iter) {
+    NH_TRY(iterate(m, true));
+    ++m;
+    ++matvecs;
+    ++skip;
+    if (o->conv_check_start < m && o->conv_check_skip <= skip) {
+      skip = 0;
+      NH_TRY(analyse(m, o));
+      done = window_done(0, o->num_eig_vb) && window_done(1, o->num_eig_cb);
+    }
+  }
+  if (!done) NH_TRY(analyse(m, o));
+
+  // the converged eigenvalues nearest the gap in each window
+  std::vector<Converged> pick;
+  const int req[2] = {o->num_eig_vb, o->num_eig_cb};
+  for (int w = 0; w < 2; ++w) {
+    const auto &r = lastRitz_[w];
+    const auto &cv = lastConv_[w];
+    int got = 0;
+    for (int t = 0; t < (int)r.size() && got < req[w]; ++t) {
+      const int idx = w == 0 ? (int)r.size() - 1 - t : t;
+      if (!cv[idx]) continue;
+      for (const auto &x : conv_)
+        if (std::fabs(x.value - r[idx]) < 100.0 * o->conv_tolerance) {
+          pick.push_back({r[idx], x.iter, w});
+          break;
+        }
+      ++got;
+    }
+  }
+  std::sort(pick.begin(), pick.end(),
+            [](const Converged &a, const Converged &b) { return a.value < b.value; });
+  const int ne = std::min((int)pick.size(), std::min(res->capacity, kMaxVec));
+  res->n_found = ne;
+  res->iterations = m;
+  for (int e = 0; e < ne; ++e) {
+    res->eigenvalues[e] = pick[e].value;
+    if (res->residuals) res->residuals[e] = -1.0;
+  }
+
+  if (o->want_vectors && ne > 0) {
+    // pass 2: rebuild the Lanczos vectors from r0 and accumulate
+    int mmax = 0;
+    for (int e = 0; e < ne; ++e) mmax = std::max(mmax, pick[e].iter);
+    std::vector<std::vector<double>> z(ne);
+    for (int e = 0; e < ne; ++e)
+      tri_eigvec(ha_, hb_, pick[e].iter, pick[e].value, z[e]);
+    double2 *V = nullptr;
+    double *dZ = nullptr;
+    NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
+    NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRing * sizeof(double)));
+    NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
+    NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
+                          hipMemcpyHostToDevice, c->stream));
+    std::vector<double> hz((size_t)kMaxVec * kRing);
+    for (int k = 0; k < mmax; ++k) {
+      NH_TRY(iterate(k, false));
+      ++matvecs;
+      if ((k + 1) % kRing == 0 || k + 1 == mmax) {
+        const int k0 = k + 1 - ((k % kRing) + 1);
+        const int K = k + 1 - k0;
+        for (int e = 0; e < ne; ++e)
+          for (int t = 0; t < K; ++t)
+            hz[e * K + t] = k0 + t < pick[e].iter ? z[e][k0 + t] : 0.0;
+        NH_HIP(hipMemcpyAsync(dZ, hz.data(), (size_t)ne * K * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+        int64_t g = (n_ + 255) / 256;
+        if (g > 8192) g = 8192;
+        // ring slots k0..k hold q_k0..q_k in order since k0 % kRing == 0
+        hipLaunchKernelGGL(k_lz_accum, dim3((int)g), dim3(256), 0, c->stream,
+                           n_, ne, K, dZ, ring_, n_, V, n_);
+        NH_HIP(hipGetLastError());
+        // the host array hz is reused next chunk: wait for the copy
+        NH_HIP(hipStreamSynchronize(c->stream));
+      }
+    }
+    // normalise, Rayleigh quotient and residual in FP64
+    for (int e = 0; e < ne; ++e) {
+      const nh_cplx *v = reinterpret_cast<const nh_cplx *>(V + (int64_t)e * n_);
+      double nrm2;
+      NH_TRY(reduce_nrm2sq(c, n_, v, &nrm2));
+      int64_t g = (n_ + 255) / 256;
+      if (g > 8192) g = 8192;
+      hipLaunchKernelGGL(k_scale, dim3((int)g), dim3(256), 0, c->stream, n_,
+                         V + (int64_t)e * n_, 1.0 / std::sqrt(nrm2));
+      NH_TRY(launch_hv(c, 0.0, v, reinterpret_cast<nh_cplx *>(Y_), 1, n_));
+      ++matvecs;
+      double rq[2];
+      NH_TRY(reduce_dot(c, n_, v, reinterpret_cast<nh_cplx *>(Y_), rq));
+      const nh_cplx a = {-rq[0], 0.0}, b = {1.0, 0.0};
+      NH_TRY(launch_axpby(c, n_, a, v, b, reinterpret_cast<nh_cplx *>(Y_)));
+      double r2;
+      NH_TRY(reduce_nrm2sq(c, n_, reinterpret_cast<nh_cplx *>(Y_), &r2));
+      res->eigenvalues[e] = rq[0];
+      if (res->residuals) res->residuals[e] = std::sqrt(r2);
+      if (res->eigenvectors)
+        NH_HIP(hipMemcpy(res->eigenvectors + (int64_t)e * n_, v,
+                         n_ * sizeof(nh_cplx), hipMemcpyDeviceToHost));
+    }
+  }
+  NH_HIP(hipStreamSynchronize(c->stream));
+  res->matvecs = matvecs;
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return done ? NH_OK : NH_ERR_NOCONV;
+}
+
+}  // namespace
+}  // namespace nh
+
+extern "C" nh_status nh_eigensolve_lanczos(nh_context *ctx,
+                                           const nh_lanczos_opts *opts,
+                                           nh_eig_result *res) {
+  NH_CHECK(ctx && opts && res && res->eigenvalues && res->capacity > 0,
+           NH_ERR_ARG, "nh_eigensolve_lanczos: bad arguments");
+  (void)hipGetLastError();
+  NH_HIP(hipSetDevice(ctx->device));
+  nh::LanczosRun run(ctx);
+  return run.run(opts, res);
+}

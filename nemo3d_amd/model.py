@@ -29,7 +29,9 @@ class TBModel:
         # diagonal (Phys.PotentialFile, ham3d_final.c:913-927)
         self.phi = None if phi is None else np.asarray(phi, dtype=np.float64)
         # structure element index -> table species index
-        remap = np.array([self.tables.index(e) for e in structure.elements],
+        used = set(np.unique(structure.species).tolist())
+        remap = np.array([self.tables.index(e) if k in used else -1
+                          for k, e in enumerate(structure.elements)],
                          dtype=np.int32)
         self.species = remap[structure.species]
 

@@ -1,0 +1,182 @@
+"""ctypes binding of libnemohip.so (the HIP implementation).
+
+There is no CPU fallback: if the library is missing or the device cannot
+be used, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnemohip.so")
+
+NH_OK = 0
+LAYOUT_BLOCKS, LAYOUT_SK, LAYOUT_MATRIX_FREE = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+PTR_HOST, PTR_DEVICE = 0, 1
+
+_STATUS = {0: "NH_OK", 1: "NH_ERR_ARG", 2: "NH_ERR_HIP", 3: "NH_ERR_STATE",
+           4: "NH_ERR_NOMEM", 5: "NH_ERR_NOCONV"}
+
+
+class NHError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (_STATUS.get(status, status), msg))
+        self.status = status
+
+
+class NHModel(ctypes.Structure):
+    _fields_ = [
+        ("nb", ctypes.c_int32),
+        ("n_atoms", ctypes.c_int32),
+        ("n_species", ctypes.c_int32),
+        ("parmat", ctypes.c_void_p),
+        ("strnmat", ctypes.c_void_p),
+        ("species", ctypes.c_void_p),
+        ("is_cation", ctypes.c_void_p),
+        ("nbr", ctypes.c_void_p),
+        ("bond_q", ctypes.c_void_p),
+        ("disp", ctypes.c_void_p),
+        ("lattice", ctypes.c_double * 3),
+        ("passivate", ctypes.c_int32),
+        ("db_shift", ctypes.c_double),
+        ("phi", ctypes.c_void_p),
+    ]
+
+
+class NHCplx(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class NHLanczosOpts(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", ctypes.c_int32),
+        ("conv_tolerance", ctypes.c_double),
+        ("num_eig_vb", ctypes.c_int32),
+        ("num_eig_cb", ctypes.c_int32),
+        ("erange_vb", ctypes.c_double * 2),
+        ("erange_cb", ctypes.c_double * 2),
+        ("conv_check_start", ctypes.c_int32),
+        ("conv_check_skip", ctypes.c_int32),
+        ("start", ctypes.c_int32),
+        ("cyclic_len", ctypes.c_int32),
+        ("cyclic_string", ctypes.c_void_p),
+        ("start_vector", ctypes.c_void_p),
+        ("resid_tol", ctypes.c_double),
+        ("want_vectors", ctypes.c_int32),
+    ]
+
+
+class NHEigResult(ctypes.Structure):
+    _fields_ = [
+        ("n_found", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("matvecs", ctypes.c_int32),
+        ("eigenvalues", ctypes.c_void_p),
+        ("residuals", ctypes.c_void_p),
+        ("eigenvectors", ctypes.c_void_p),
+        ("capacity", ctypes.c_int32),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+# Every symbol declared in include/nemohip.h: name -> (restype, argtypes)
+P = ctypes.c_void_p
+SIGNATURES = {
+    "nh_context_create": (ctypes.c_int, [P, ctypes.c_int, P]),
+    "nh_context_destroy": (None, [P]),
+    "nh_context_dim": (ctypes.c_int64, [P]),
+    "nh_ham_build": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "nh_hv": (ctypes.c_int, [P, ctypes.c_double, P, P, ctypes.c_int]),
+    "nh_hV": (ctypes.c_int, [P, ctypes.c_double, P, P, ctypes.c_int,
+                             ctypes.c_int64, ctypes.c_int]),
+    "nh_ham_export": (ctypes.c_int, [P, P, P]),
+    "nh_device_alloc": (ctypes.c_int, [P, ctypes.c_int64, P]),
+    "nh_device_free": (ctypes.c_int, [P, P]),
+    "nh_memcpy_h2d": (ctypes.c_int, [P, P, P, ctypes.c_int64]),
+    "nh_memcpy_d2h": (ctypes.c_int, [P, P, P, ctypes.c_int64]),
+    "nh_dot": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
+    "nh_nrm2": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
+    "nh_axpby": (ctypes.c_int, [P, ctypes.c_int64, NHCplx, P, NHCplx, P]),
+    "nh_eigensolve_lanczos": (ctypes.c_int, [P, P, P]),
+    "nh_last_error": (ctypes.c_char_p, []),
+}
+
+_LIB = None
+
+
+def load():
+    """Load libnemohip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libnemohip.so not built: run "
+                               "`python -m nemo3d_amd.build` (HIP path only, "
+                               "no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(status):
+    if status != NH_OK:
+        raise NHError(status, load().nh_last_error().decode())
+
+
+def ptr(a):
+    """Address of a numpy array (host) or a DeviceArray (device)."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(ctypes.c_void_p)
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    raise TypeError(type(a))
+
+
+class DeviceArray:
+    """Device allocation made through a libnemohip context.
+
+    `owner` is the object holding the context (it must expose `.ctx`); the
+    array keeps a reference to it so the context outlives the buffer."""
+
+    def __init__(self, owner, shape, dtype=np.complex128):
+        self.owner = owner
+        ctx = owner.ctx
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        check(load().nh_device_alloc(ctx, self.nbytes, ctypes.byref(self.ptr)))
+
+    @classmethod
+    def from_host(cls, owner, a):
+        a = np.ascontiguousarray(a)
+        d = cls(owner, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(load().nh_memcpy_h2d(self.ctx, self.ptr, ptr(a), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(load().nh_memcpy_d2h(self.ctx, ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(load().nh_device_free(self.ctx, self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,60 @@
+"""Solver drivers on the device Hamiltonian.
+
+`eigensolve_lanczos` keeps the knobs of the reference's Lanczos deck block
+(``Lanczos_struct``: MaxIter, ConvTolerance, NumEigReq_vb/cb, Erange_vb/cb,
+ConvCheckStartIter, ConvCheckSkipRate, Start, CyclicString) and the
+semantics of ``eigensolve_lanczos`` (NEMO_3D/eigen/src/lanczos.c:1823-2089);
+the iteration runs in libnemohip on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import native
+
+DEFAULT_CYCLIC = (1.2, 0.8, -1.4, 0.1, -0.9, -0.2, 0.6)
+START = {"CyclicRandom": 0, "Const1": 1, "Vector": 2}
+
+
+def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
+                       erange_cb=(1.4, 2.0), max_iter=8000, tol=1e-7,
+                       start_iter=200, skip=50, start="CyclicRandom",
+                       cyclic=DEFAULT_CYCLIC, start_vector=None,
+                       want_vectors=True, capacity=64, return_vectors=False):
+    lib = native.load()
+    o = native.NHLanczosOpts()
+    o.max_iter = int(max_iter)
+    o.conv_tolerance = float(tol)
+    o.num_eig_vb = int(num_vb)
+    o.num_eig_cb = int(num_cb)
+    o.erange_vb[0], o.erange_vb[1] = map(float, erange_vb)
+    o.erange_cb[0], o.erange_cb[1] = map(float, erange_cb)
+    o.conv_check_start = int(start_iter)
+    o.conv_check_skip = int(skip)
+    o.start = START[start if start_vector is None else "Vector"]
+    cyc = np.ascontiguousarray(cyclic, dtype=np.float64)
+    o.cyclic_len = cyc.size
+    o.cyclic_string = native.ptr(cyc)
+    if start_vector is not None:
+        sv = np.ascontiguousarray(start_vector, dtype=np.complex128)
+        o.start_vector = native.ptr(sv)
+    o.resid_tol = 0.0
+    o.want_vectors = int(bool(want_vectors))
+    ev = np.zeros(capacity)
+    rs = np.full(capacity, -1.0)
+    vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
+    r = native.NHEigResult()
+    r.eigenvalues = native.ptr(ev)
+    r.residuals = native.ptr(rs)
+    r.eigenvectors = native.ptr(vec) if vec is not None else None
+    r.capacity = capacity
+    st = lib.nh_eigensolve_lanczos(H.ctx, ctypes.byref(o), ctypes.byref(r))
+    if st not in (native.NH_OK, 5):
+        native.check(st)
+    n = r.n_found
+    out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
+           "iterations": r.iterations, "matvecs": r.matvecs,
+           "seconds": r.seconds, "converged": st == native.NH_OK}
+    if vec is not None:
+        out["vectors"] = vec[:n]
+    return out

@@ -1,0 +1,114 @@
+"""H·v / H·V on the GPU through the C ABI against the CPU oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import _models
+from nemo3d_amd import native
+from nemo3d_amd.hamiltonian import Hamiltonian
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "gaas20_strained": dict(builder="gaas", cells=(3, 2, 2), strain_nm=0.02, seed=3),
+    "ingaas20_alloy_phi": dict(builder="gaas", cells=(2, 3, 2), strain_nm=0.015,
+                               seed=4, alloy=True, phi=True),
+    "gaas20_unpassivated": dict(builder="gaas", cells=(2, 2, 2), seed=5,
+                                passivate=False),
+    "si10_strained": dict(builder="si", cells=(3, 3, 3), strain_nm=0.02, seed=6),
+}
+
+
+def make(case):
+    kw = dict(CASES[case])
+    b = kw.pop("builder")
+    return _models.gaas_box(**kw) if b == "gaas" else _models.si_box(**kw)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_assembly_matches_oracle(case):
+    M = make(case)
+    print("case", case, "atoms", M.n_atoms, flush=True)
+    ref = oracle.assemble(M.arrays())
+    H = Hamiltonian(M)
+    onsite, blocks = H.export()
+    scale = np.abs(ref.onsite[:, :10]).max()
+    assert np.abs(onsite - ref.onsite).max() <= 1e-12 * scale
+    bscale = np.abs(ref.blocks).max()
+    assert np.abs(blocks - ref.blocks).max() <= 1e-12 * bscale
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("shift", [0.0, 1.3])
+def test_hv_fp64_matches_oracle(case, shift):
+    M = make(case)
+    print("case", case, "shift", shift, flush=True)
+    ref = oracle.assemble(M.arrays())
+    H = Hamiltonian(M)
+    X = _models.random_vectors(H.n, 3, seed=7)
+    for x in X:
+        y = H.hv(x, shift)
+        yr = ref.hv(x, shift)
+        assert np.linalg.norm(y - yr) <= 1e-12 * np.linalg.norm(yr)
+
+
+@pytest.mark.parametrize("case", ["gaas20_strained", "si10_strained"])
+def test_hV_block_equals_columns(case):
+    M = make(case)
+    H = Hamiltonian(M)
+    X = _models.random_vectors(H.n, 8, seed=8)
+    Y = H.hV(X, 0.4)
+    for k in range(8):
+        assert np.array_equal(Y[k], H.hv(X[k], 0.4))
+
+
+@pytest.mark.parametrize("case", ["gaas20_strained", "si10_strained"])
+def test_hv_fp32_error_bound(case):
+    """FP32 stored values with FP32 products per block: error relative to
+    the FP64 product stays at single-precision level."""
+    M = make(case)
+    ref = oracle.assemble(M.arrays())
+    H = Hamiltonian(M, precision=native.PREC_FP32)
+    X = _models.random_vectors(H.n, 2, seed=9)
+    for x in X:
+        y = H.hv(x, 0.0)
+        yr = ref.hv(x, 0.0)
+        err = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+        assert 1e-10 < err < 2e-6, err
+
+
+def test_vector_kernels():
+    M = make("si10_strained")
+    H = Hamiltonian(M)
+    lib = native.load()
+    rng = np.random.default_rng(2)
+    for n in (1, 17, 1000, 300001):
+        x = rng.normal(size=n) + 1j * rng.normal(size=n)
+        y = rng.normal(size=n) + 1j * rng.normal(size=n)
+        tx = native.DeviceArray.from_host(H, x)
+        ty = native.DeviceArray.from_host(H, y)
+        out = native.NHCplx()
+        native.check(lib.nh_dot(H.ctx, n, native.ptr(tx), native.ptr(ty),
+                                ctypes.byref(out)))
+        ref = np.vdot(x, y)
+        assert abs(complex(out.re, out.im) - ref) <= 1e-12 * n
+        nrm = ctypes.c_double()
+        native.check(lib.nh_nrm2(H.ctx, n, native.ptr(tx), ctypes.byref(nrm)))
+        assert abs(nrm.value - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+        a = native.NHCplx(0.5, -1.25)
+        b = native.NHCplx(-2.0, 0.75)
+        native.check(lib.nh_axpby(H.ctx, n, a, native.ptr(tx), b, native.ptr(ty)))
+        want = (0.5 - 1.25j) * x + (-2.0 + 0.75j) * y
+        assert np.allclose(ty.download(), want, rtol=1e-14, atol=1e-14)
+
+
+def test_device_buffers_hv():
+    """H·v on device-resident buffers equals the host-pointer path."""
+    M = make("gaas20_strained")
+    H = Hamiltonian(M)
+    X = _models.random_vectors(H.n, 4, seed=11)
+    dX = native.DeviceArray.from_host(H, X)
+    dY = H.hV(dX, 0.7)
+    assert np.array_equal(dY.download(), H.hV(X, 0.7))
