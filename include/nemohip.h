@@ -154,6 +154,15 @@ typedef struct nh_eig_result {
   nh_cplx *eigenvectors; /* caller host array capacity*N, or NULL */
   int32_t capacity;
   double seconds;        /* wall time of the solve */
+  double hv_ms;          /* mean duration of the fused H·v kernel (HIP
+                            events on the solver stream, sampled launches) */
+  int32_t hv_timed;      /* number of timed H·v launches */
+  /* optional: the tridiagonal of the eigenvalue pass (alpha[k], beta[k]),
+     caller arrays of capacity tridiag_cap, like the reference's saved
+     nd_tridiag_bin (lanczos.c:1692-1809) */
+  double *tridiag_alpha;
+  double *tridiag_beta;
+  int32_t tridiag_cap;
 } nh_eig_result;
 
 /* Lanczos without reorthogonalisation, eigenvalues from the tridiagonal
@@ -161,6 +170,24 @@ typedef struct nh_eig_result {
  * (eigensolve_lanczos, NEMO_3D/eigen/src/lanczos.c:1823-2089). */
 nh_status nh_eigensolve_lanczos(nh_context *ctx, const nh_lanczos_opts *opts,
                                 nh_eig_result *res);
+
+/* ---- valence force field ------------------------------------------------
+ * Keating VFF (calc_estrain / min_estrain, strain3d.c:157-475,1361-2245):
+ * alpha, beta, a0 are [S][S] tables (Keating constants and unstrained
+ * lattice constant of each species pair).  Atoms with fewer than four
+ * neighbours stay fixed. */
+/* Energy and gradient at displacements disp[A][3] (host arrays). */
+nh_status nh_vff_energy_grad(nh_context *ctx, const double *alpha,
+                             const double *beta, const double *a0,
+                             const double *disp, double *grad_out,
+                             double *energy);
+/* Relax the context's displacements in place by conjugate gradients until
+ * ||grad|| <= gtol or max_iter; disp_out (host [A][3], may be NULL)
+ * receives the result; info[5] = {iterations, E0, E, |g0|, |g|}.  Rebuild
+ * the Hamiltonian afterwards. */
+nh_status nh_vff_relax(nh_context *ctx, const double *alpha,
+                       const double *beta, const double *a0, int max_iter,
+                       double gtol, double *disp_out, double *info);
 
 const char *nh_last_error(void);
 

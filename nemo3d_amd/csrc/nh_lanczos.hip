@@ -236,6 +236,7 @@ class LanczosRun {
  public:
   LanczosRun(nh_context *c) : c_(c) {}
   ~LanczosRun() {
+    release_events();
     for (void *p : allocs_) hipFree(p);
   }
   nh_status run(const nh_lanczos_opts *o, nh_eig_result *res);
@@ -247,6 +248,7 @@ class LanczosRun {
     return NH_OK;
   }
   nh_status iterate(int k, bool with_reductions);
+  nh_status collect_timing();
   nh_status analyse(int m, const nh_lanczos_opts *o);
   bool window_done(int w, int req) const;
 
@@ -260,7 +262,35 @@ class LanczosRun {
   std::vector<Converged> conv_;
   std::vector<double> lastRitz_[2];
   std::vector<char> lastConv_[2];
+  // sampled H·v timing: one event pair every kTimeEvery iterations
+  static constexpr int kTimeEvery = 16;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_;
+  double hv_ms_sum_ = 0.0;
+  int hv_timed_ = 0;
+
+ public:
+  double hv_ms() const { return hv_timed_ ? hv_ms_sum_ / hv_timed_ : 0.0; }
+  int hv_timed() const { return hv_timed_; }
+  void release_events() {
+    for (auto &e : ev_) {
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
+    ev_.clear();
+  }
 };
+
+nh_status LanczosRun::collect_timing() {
+  NH_HIP(hipStreamSynchronize(c_->stream));
+  for (auto &e : ev_) {
+    float ms = 0.f;
+    NH_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    hv_ms_sum_ += ms;
+    ++hv_timed_;
+  }
+  release_events();
+  return NH_OK;
+}
 
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
@@ -269,6 +299,13 @@ nh_status LanczosRun::iterate(int k, bool red) {
   double2 *Qp = k > 0 ? ring_ + (int64_t)((k - 1) % kRing) * n_ : nullptr;
   const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
   double *part = red ? part_ : nullptr;
+  const bool timed = red && (k % kTimeEvery) == 0;
+  std::pair<hipEvent_t, hipEvent_t> evp;
+  if (timed) {
+    NH_HIP(hipEventCreate(&evp.first));
+    NH_HIP(hipEventCreate(&evp.second));
+    NH_HIP(hipEventRecord(evp.first, c->stream));
+  }
 #define LZ_HV(NSP, TB, BLK)                                                   \
   hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB>), dim3(gridA), dim3(kThreads), 0,  \
                      c->stream, c->n_atoms, c->onsite, BLK, c->nbr, R_, Q, Y_, \
@@ -281,6 +318,10 @@ nh_status LanczosRun::iterate(int k, bool red) {
     else LZ_HV(1, float, c->blocks32);
   }
 #undef LZ_HV
+  if (timed) {
+    NH_HIP(hipEventRecord(evp.second, c->stream));
+    ev_.push_back(evp);
+  }
   if (red)
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        gridA, 1, part_, alpha_ + k, 0);
@@ -321,6 +362,7 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
                         (m - have) * sizeof(double), hipMemcpyDeviceToHost,
                         c_->stream));
   NH_HIP(hipStreamSynchronize(c_->stream));
+  NH_TRY(collect_timing());
   const double tol = o->conv_tolerance;
   const double resolution = 100.0 * tol;  // lanczos.c:1970
   const double bias = 1e-80;
@@ -422,6 +464,11 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     }
   }
   if (!done) NH_TRY(analyse(m, o));
+  if (res->tridiag_alpha && res->tridiag_beta) {
+    const int cnt = std::min(m, res->tridiag_cap);
+    std::copy(ha_.begin(), ha_.begin() + cnt, res->tridiag_alpha);
+    std::copy(hb_.begin(), hb_.begin() + cnt, res->tridiag_beta);
+  }
 
   // the converged eigenvalues nearest the gap in each window
   std::vector<Converged> pick;
@@ -432,7 +479,10 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     int got = 0;
     for (int t = 0; t < (int)r.size() && got < req[w]; ++t) {
       const int idx = w == 0 ? (int)r.size() - 1 - t : t;
-      if (!cv[idx]) continue;
+      // report only the converged prefix nearest the gap: an unconverged
+      // Ritz value there may be a state that a farther converged one would
+      // wrongly replace
+      if (!cv[idx]) break;
       for (const auto &x : conv_)
         if (std::fabs(x.value - r[idx]) < 100.0 * o->conv_tolerance) {
           pick.push_back({r[idx], x.iter, w});
@@ -513,6 +563,9 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   }
   NH_HIP(hipStreamSynchronize(c->stream));
   res->matvecs = matvecs;
+  NH_TRY(collect_timing());
+  res->hv_ms = hv_ms();
+  res->hv_timed = hv_timed();
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return done ? NH_OK : NH_ERR_NOCONV;
 }

@@ -78,6 +78,11 @@ class NHEigResult(ctypes.Structure):
         ("eigenvectors", ctypes.c_void_p),
         ("capacity", ctypes.c_int32),
         ("seconds", ctypes.c_double),
+        ("hv_ms", ctypes.c_double),
+        ("hv_timed", ctypes.c_int32),
+        ("tridiag_alpha", ctypes.c_void_p),
+        ("tridiag_beta", ctypes.c_void_p),
+        ("tridiag_cap", ctypes.c_int32),
     ]
 
 
@@ -100,6 +105,9 @@ SIGNATURES = {
     "nh_nrm2": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "nh_axpby": (ctypes.c_int, [P, ctypes.c_int64, NHCplx, P, NHCplx, P]),
     "nh_eigensolve_lanczos": (ctypes.c_int, [P, P, P]),
+    "nh_vff_energy_grad": (ctypes.c_int, [P, P, P, P, P, P, P]),
+    "nh_vff_relax": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_double,
+                                    P, P]),
     "nh_last_error": (ctypes.c_char_p, []),
 }
 

@@ -20,7 +20,8 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                        erange_cb=(1.4, 2.0), max_iter=8000, tol=1e-7,
                        start_iter=200, skip=50, start="CyclicRandom",
                        cyclic=DEFAULT_CYCLIC, start_vector=None,
-                       want_vectors=True, capacity=64, return_vectors=False):
+                       want_vectors=True, capacity=64, return_vectors=False,
+                       return_tridiag=False):
     lib = native.load()
     o = native.NHLanczosOpts()
     o.max_iter = int(max_iter)
@@ -48,13 +49,23 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     r.residuals = native.ptr(rs)
     r.eigenvectors = native.ptr(vec) if vec is not None else None
     r.capacity = capacity
+    if return_tridiag:
+        ta = np.zeros(max_iter + 1)
+        tb = np.zeros(max_iter + 1)
+        r.tridiag_alpha = native.ptr(ta)
+        r.tridiag_beta = native.ptr(tb)
+        r.tridiag_cap = max_iter + 1
     st = lib.nh_eigensolve_lanczos(H.ctx, ctypes.byref(o), ctypes.byref(r))
     if st not in (native.NH_OK, 5):
         native.check(st)
     n = r.n_found
     out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
            "iterations": r.iterations, "matvecs": r.matvecs,
-           "seconds": r.seconds, "converged": st == native.NH_OK}
+           "seconds": r.seconds, "converged": st == native.NH_OK,
+           "hv_ms": r.hv_ms, "hv_timed": r.hv_timed}
+    if return_tridiag:
+        out["alpha"] = ta[:r.iterations].copy()
+        out["beta"] = tb[:r.iterations].copy()
     if vec is not None:
         out["vectors"] = vec[:n]
     return out
