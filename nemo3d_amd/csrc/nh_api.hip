@@ -39,7 +39,7 @@ static void destroy(nh_context *c) {
   if (!c) return;
   hipSetDevice(c->device);
   void *ptrs[] = {c->parmat, c->strnmat, c->species, c->is_cation, c->nbr,
-                  c->bond_q, c->disp, c->phi, c->cations, c->bond_of,
+                  c->bond_q, c->disp, c->phi, c->cations, c->anions, c->bond_of,
                   c->rev_slot, c->onsite, c->blocks64, c->blocks32,
                   c->onsite32, c->sk64, c->sk32, c->corr, c->red, c->xbuf,
                   c->ybuf};
@@ -59,7 +59,7 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
            NH_ERR_ARG, "missing model array");
   const int A = m->n_atoms, S = m->n_species;
   NH_CHECK((int64_t)A * m->nb < ((int64_t)1 << 40), NH_ERR_ARG, "too large");
-  std::vector<int32_t> cations, bond_of((size_t)A * 4, -1),
+  std::vector<int32_t> cations, anions, bond_of((size_t)A * 4, -1),
       rev((size_t)A * 4, -1);
   for (int i = 0; i < A; ++i) {
     NH_CHECK(m->species[i] >= 0 && m->species[i] < S, NH_ERR_ARG,
@@ -83,7 +83,7 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
     NH_CHECK(present > 0, NH_ERR_ARG, "atom %d has no neighbours", i);
   }
   for (int i = 0; i < A; ++i)
-    if (m->is_cation[i]) cations.push_back(i);
+    (m->is_cation[i] ? cations : anions).push_back(i);
   for (size_t ci = 0; ci < cations.size(); ++ci) {
     const int i = cations[ci];
     for (int n = 0; n < 4; ++n) {
@@ -102,6 +102,7 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
   c->n_atoms = A;
   c->n_species = S;
   c->n_cations = (int)cations.size();
+  c->n_anions = (int)anions.size();
   c->n = (int64_t)A * m->nb;
   for (int k = 0; k < 3; ++k) c->lattice[k] = m->lattice[k];
   c->passivate = m->passivate;
@@ -118,6 +119,7 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
   UP(disp, m->disp, (size_t)A * 3);
   if (m->phi) UP(phi, m->phi, A);
   UP(cations, cations.data(), cations.size());
+  if (!anions.empty()) UP(anions, anions.data(), anions.size());
   UP(bond_of, bond_of.data(), (size_t)A * 4);
   UP(rev_slot, rev.data(), (size_t)A * 4);
 #undef UP

@@ -1,9 +1,46 @@
 // Sparse H·v and block H·V for the stored-block layout (row computation
 // in nh_hv_row.h).
 #include "nh_hv_row.h"
+#include "nh_hv_sk.h"
 
 namespace nh {
 namespace {
+
+// SK-compressed layout (see nh_hv_sk.h).  Workgroups [0, nbc) take cation
+// atoms, the rest anion atoms.
+template <int NSP, typename R>
+__global__ __launch_bounds__(kSkThreads) void k_hv_sk(
+    int nc, const int32_t *__restrict__ cations, int na,
+    const int32_t *__restrict__ anions, int nbc,
+    const double *__restrict__ onsite, const R *__restrict__ sk,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
+    double shift, const double2 *__restrict__ x, double2 *__restrict__ y,
+    int64_t ld) {
+  constexpr int NB = NSP * 10;
+  const bool cat = (int)blockIdx.x < nbc;
+  const int bl = cat ? blockIdx.x : blockIdx.x - nbc;
+  const int li = bl * kSkAtomsPerBlock + threadIdx.x / 4;
+  const int slot = threadIdx.x & 3;
+  const bool active = li < (cat ? nc : na);
+  const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
+  const double2 *xc = x + (int64_t)blockIdx.y * ld;
+  double2 *yc = y + (int64_t)blockIdx.y * ld;
+  double ar[NB], ai[NB];
+  if (cat)
+    sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, xc, ar, ai);
+  else
+    sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, xc, ar, ai);
+  if (!active) return;
+  const double *os = onsite + (size_t)atom * kOnsiteStride;
+  const double2 *xa = xc + (int64_t)atom * NB;
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    if ((r & 3) != slot) continue;
+    double re = ar[r], im = ai[r];
+    onsite_row<NSP>(os, xa, r / 10, r % 10, shift, re, im);
+    yc[(int64_t)atom * NB + r] = make_double2(re, im);
+  }
+}
 
 template <int NSP, typename TB, typename TA>
 __global__ __launch_bounds__(kThreads) void k_hv_blocks(
@@ -30,10 +67,31 @@ __global__ __launch_bounds__(kThreads) void k_hv_blocks(
 
 nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
                     int ncol, int64_t ld) {
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS, NH_ERR_STATE,
-           "H·v: Hamiltonian not built in a supported layout");
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
+           NH_ERR_STATE, "H·v: Hamiltonian not built in a supported layout");
   NH_CHECK(ncol >= 1 && ncol <= 65535 && ld >= c->n, NH_ERR_ARG,
            "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
+  if (c->layout == NH_LAYOUT_SK) {
+    const int nbc = (c->n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+    const int nba = (c->n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+    const dim3 g(nbc + nba, ncol);
+    const double2 *xd = reinterpret_cast<const double2 *>(x);
+    double2 *yd = reinterpret_cast<double2 *>(y);
+#define SKL(NSP, R, SKP)                                                     \
+  hipLaunchKernelGGL((k_hv_sk<NSP, R>), g, dim3(kSkThreads), 0, c->stream,   \
+                     c->n_cations, c->cations, c->n_anions, c->anions, nbc,  \
+                     c->onsite, SKP, c->nbr, c->bond_of, shift, xd, yd, ld)
+    if (c->precision == NH_PREC_FP64) {
+      if (c->nb == 20) SKL(2, double, c->sk64);
+      else SKL(1, double, c->sk64);
+    } else {
+      if (c->nb == 20) SKL(2, float, c->sk32);
+      else SKL(1, float, c->sk32);
+    }
+#undef SKL
+    NH_HIP(hipGetLastError());
+    return NH_OK;
+  }
   const dim3 grid((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock, ncol);
   const double2 *xd = reinterpret_cast<const double2 *>(x);
   double2 *yd = reinterpret_cast<double2 *>(y);

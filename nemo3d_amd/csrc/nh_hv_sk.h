@@ -1,0 +1,163 @@
+// H·v in the SK-compressed layout: per bond only the direction cosines and
+// the 21 strain-scaled two-centre integrals are stored (24 values, each
+// bond once, cation-owned); the 10x10 block is rebuilt row by row in
+// registers on every product.  The onsite record (diagonal with Boykin
+// corrections, surface couplings, spin-orbit) is stored per atom as in the
+// block layout.
+//
+// Mapping: 4 lanes per atom, lane n owns neighbour slot n.  A workgroup
+// holds only cation-role or only anion-role atoms (lists `cations` /
+// `anions`), so whether the lane applies T (anion row) or its transpose
+// (cation row) is uniform across the wavefront.  Each lane forms its bond's
+// 20 (10) complex products with R-precision arithmetic inside the block,
+// adds them to FP64 accumulators, a 4-lane butterfly sums the four bonds,
+// and lane n then finishes and writes rows r = n, n+4, ... with the onsite
+// terms.
+#pragma once
+
+#include "nh_internal.h"
+#include "nh_sk_row.h"
+
+namespace nh {
+
+constexpr int kSkThreads = 256;
+constexpr int kSkAtomsPerBlock = kSkThreads / 4;
+
+// Onsite contribution (diagonal - shift, surface s-p couplings, spin-orbit)
+// to row (s, o) of atom with vector base pointer xa.
+template <int NSP>
+__device__ __forceinline__ void onsite_row(const double *__restrict__ os,
+                                           const double2 *__restrict__ xa,
+                                           int s, int o, double shift,
+                                           double &re, double &im) {
+  const double2 xo = xa[s * 10 + o];
+  const double d = os[o] - shift;
+  re += d * xo.x;
+  im += d * xo.y;
+  if (o >= 1 && o <= 4) {
+    // partners and stored coupling index: (1,2)0 (1,3)1 (1,4)2 (2,3)3 (2,4)4 (3,4)5
+    const int p0 = o == 1 ? 2 : 1, k0 = o == 1 ? 0 : o == 2 ? 0 : o == 3 ? 1 : 2;
+    const int p1 = o <= 2 ? 3 : 2, k1 = o == 1 ? 1 : o == 2 ? 3 : o == 3 ? 3 : 4;
+    const int p2 = o <= 3 ? 4 : 3, k2 = o == 1 ? 2 : o == 2 ? 4 : 5;
+    const double h0 = os[10 + k0], h1 = os[10 + k1], h2 = os[10 + k2];
+    const double2 a0 = xa[s * 10 + p0], a1 = xa[s * 10 + p1], a2 = xa[s * 10 + p2];
+    re += h0 * a0.x + h1 * a1.x + h2 * a2.x;
+    im += h0 * a0.y + h1 * a1.y + h2 * a2.y;
+  }
+  if (NSP == 2 && o >= 2 && o <= 4) {
+    const double lam = os[16];
+    if (s == 0) {
+      if (o == 2) {
+        re += lam * xa[3].y + lam * xa[14].x;
+        im += -lam * xa[3].x + lam * xa[14].y;
+      } else if (o == 3) {
+        re += -lam * xa[2].y + lam * xa[14].y;
+        im += lam * xa[2].x - lam * xa[14].x;
+      } else {
+        re += -lam * xa[12].x - lam * xa[13].y;
+        im += -lam * xa[12].y + lam * xa[13].x;
+      }
+    } else {
+      if (o == 2) {
+        re += -lam * xa[13].y - lam * xa[4].x;
+        im += lam * xa[13].x - lam * xa[4].y;
+      } else if (o == 3) {
+        re += lam * xa[12].y + lam * xa[4].y;
+        im += -lam * xa[12].x - lam * xa[4].x;
+      } else {
+        re += lam * xa[2].x - lam * xa[3].y;
+        im += lam * xa[2].y + lam * xa[3].x;
+      }
+    }
+  }
+}
+
+// Block product of one bond, rows applied by orientation: anion rows
+// acc[s][a] += sum_c T[a][c] x[s][c]; cation rows acc[s][c] += sum_a
+// T[a][c] x[s][a].
+template <int NSP, typename R, bool CATION>
+struct BondApply {
+  template <int A>
+  __device__ __forceinline__ static void run(R l, R m, R n, const R *v,
+                                             const R (&xr)[NSP][10],
+                                             const R (&xi)[NSP][10],
+                                             R (&br)[NSP][10],
+                                             R (&bi)[NSP][10]) {
+    if constexpr (A < 10) {
+      R row[10];
+      sk_row<A, R>(l, m, n, v, row);
+#pragma unroll
+      for (int s = 0; s < NSP; ++s) {
+        if constexpr (!CATION) {
+          R sr = R(0), si = R(0);
+#pragma unroll
+          for (int c = 0; c < 10; ++c) {
+            sr += row[c] * xr[s][c];
+            si += row[c] * xi[s][c];
+          }
+          br[s][A] += sr;
+          bi[s][A] += si;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 10; ++c) {
+            br[s][c] += row[c] * xr[s][A];
+            bi[s][c] += row[c] * xi[s][A];
+          }
+        }
+      }
+      run<A + 1>(l, m, n, v, xr, xi, br, bi);
+    }
+  }
+};
+
+// Accumulates the 4 bonds of `atom` (lane = slot) into acc, summed over the
+// 4 lanes; returns with every lane of the quad holding the full sums.
+template <int NSP, typename R, bool CATION>
+__device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
+                                         const R *__restrict__ sk,
+                                         const int32_t *__restrict__ nbr,
+                                         const int32_t *__restrict__ bond_of,
+                                         const double2 *__restrict__ xc,
+                                         double (&ar)[NSP * 10],
+                                         double (&ai)[NSP * 10]) {
+  constexpr int NB = NSP * 10;
+#pragma unroll
+  for (int r = 0; r < NB; ++r) ar[r] = ai[r] = 0.0;
+  const int j = active ? nbr[atom * 4 + slot] : -1;
+  if (j >= 0) {
+    const R *rec = sk + (size_t)bond_of[atom * 4 + slot] * kSkStride;
+    R v[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) v[k] = rec[3 + k];
+    const R l = rec[0], m = rec[1], n = rec[2];
+    R xr[NSP][10], xi[NSP][10], br[NSP][10], bi[NSP][10];
+    const double2 *xj = xc + (int64_t)j * NB;
+#pragma unroll
+    for (int s = 0; s < NSP; ++s)
+#pragma unroll
+      for (int c = 0; c < 10; ++c) {
+        const double2 t = xj[s * 10 + c];
+        xr[s][c] = R(t.x);
+        xi[s][c] = R(t.y);
+        br[s][c] = bi[s][c] = R(0);
+      }
+    BondApply<NSP, R, CATION>::template run<0>(l, m, n, v, xr, xi, br, bi);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s)
+#pragma unroll
+      for (int c = 0; c < 10; ++c) {
+        ar[s * 10 + c] = double(br[s][c]);
+        ai[s * 10 + c] = double(bi[s][c]);
+      }
+  }
+  // sum over the quad (lanes 4k..4k+3); fixed order, every lane gets it
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    ar[r] += __shfl_xor(ar[r], 1, 64);
+    ai[r] += __shfl_xor(ai[r], 1, 64);
+    ar[r] += __shfl_xor(ar[r], 2, 64);
+    ai[r] += __shfl_xor(ai[r], 2, 64);
+  }
+}
+
+}  // namespace nh

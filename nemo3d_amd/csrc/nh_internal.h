@@ -51,6 +51,7 @@ struct nh_context {
   int n_atoms = 0;
   int n_species = 0;
   int n_cations = 0;
+  int n_anions = 0;
   int64_t n = 0;     // n_atoms * nb
   double lattice[3] = {0, 0, 0};
   int passivate = 0;
@@ -67,6 +68,7 @@ struct nh_context {
   double *disp = nullptr;       // [A][3]
   double *phi = nullptr;        // [A] or null
   int32_t *cations = nullptr;   // [Nc] atom index of each cation
+  int32_t *anions = nullptr;    // [Na] atom index of each anion
   int32_t *bond_of = nullptr;   // [A][4] cation-owned bond id (ci*4+slot)
   int32_t *rev_slot = nullptr;  // [A][4] slot of this atom in the nbr's list
 

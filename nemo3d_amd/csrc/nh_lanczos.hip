@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "nh_hv_row.h"
+#include "nh_hv_sk.h"
 #include "nh_vector.h"
 
 namespace nh {
@@ -72,6 +73,54 @@ __global__ __launch_bounds__(kThreads) void k_lz_hv(
   if (part) {
     const double t = wg_sum320(local);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+// SK-layout variant of k_lz_hv (mapping of nh_hv_sk.h: 4 lanes per atom,
+// cation and anion workgroups).
+template <int NSP, typename R>
+__global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
+    int nc, const int32_t *__restrict__ cations, int na,
+    const int32_t *__restrict__ anions, int nbc,
+    const double *__restrict__ onsite, const R *__restrict__ sk,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
+    const double2 *__restrict__ Rv, double2 *__restrict__ Q,
+    double2 *__restrict__ Y, const double *__restrict__ beta_prev,
+    double *__restrict__ part) {
+  constexpr int NB = NSP * 10;
+  const bool cat = (int)blockIdx.x < nbc;
+  const int bl = cat ? blockIdx.x : blockIdx.x - nbc;
+  const int li = bl * kSkAtomsPerBlock + threadIdx.x / 4;
+  const int slot = threadIdx.x & 3;
+  const bool active = li < (cat ? nc : na);
+  const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
+  const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
+  double ar[NB], ai[NB];
+  if (cat)
+    sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, Rv, ar, ai);
+  else
+    sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, Rv, ar, ai);
+  double local = 0.0;
+  if (active) {
+    const double *os = onsite + (size_t)atom * kOnsiteStride;
+    const double2 *xa = Rv + (int64_t)atom * NB;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      if ((r & 3) != slot) continue;
+      double re = ar[r], im = ai[r];
+      onsite_row<NSP>(os, xa, r / 10, r % 10, 0.0, re, im);
+      const double2 xr = xa[r];
+      const double2 q = make_double2(xr.x * inv, xr.y * inv);
+      const double2 y = make_double2(re * inv, im * inv);
+      Q[(int64_t)atom * NB + r] = q;
+      Y[(int64_t)atom * NB + r] = y;
+      local += q.x * y.x + q.y * y.y;
+    }
+  }
+  if (part) {
+    double z = 0.0;
+    block_sum2(local, z);
+    if (threadIdx.x == 0) part[blockIdx.x] = local;
   }
 }
 
@@ -294,7 +343,11 @@ nh_status LanczosRun::collect_timing() {
 
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
-  const int gridA = (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
+  const int nbc = (c->n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+  const int nba = (c->n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+  const int gridA = c->layout == NH_LAYOUT_SK
+                        ? nbc + nba
+                        : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   double2 *Q = ring_ + (int64_t)(k % kRing) * n_;
   double2 *Qp = k > 0 ? ring_ + (int64_t)((k - 1) % kRing) * n_ : nullptr;
   const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
@@ -310,7 +363,20 @@ nh_status LanczosRun::iterate(int k, bool red) {
   hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB>), dim3(gridA), dim3(kThreads), 0,  \
                      c->stream, c->n_atoms, c->onsite, BLK, c->nbr, R_, Q, Y_, \
                      bprev, part)
-  if (c->precision == NH_PREC_FP64) {
+#define LZ_SK(NSP, R, SKP)                                                      \
+  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R>), dim3(gridA), dim3(kSkThreads), 0,    \
+                     c->stream, c->n_cations, c->cations, c->n_anions,          \
+                     c->anions, nbc, c->onsite, SKP, c->nbr, c->bond_of, R_, Q, \
+                     Y_, bprev, part)
+  if (c->layout == NH_LAYOUT_SK) {
+    if (c->precision == NH_PREC_FP64) {
+      if (c->nb == 20) LZ_SK(2, double, c->sk64);
+      else LZ_SK(1, double, c->sk64);
+    } else {
+      if (c->nb == 20) LZ_SK(2, float, c->sk32);
+      else LZ_SK(1, float, c->sk32);
+    }
+  } else if (c->precision == NH_PREC_FP64) {
     if (c->nb == 20) LZ_HV(2, double, c->blocks64);
     else LZ_HV(1, double, c->blocks64);
   } else {
@@ -318,6 +384,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
     else LZ_HV(1, float, c->blocks32);
   }
 #undef LZ_HV
+#undef LZ_SK
   if (timed) {
     NH_HIP(hipEventRecord(evp.second, c->stream));
     ev_.push_back(evp);
@@ -407,8 +474,8 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
 nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   nh_context *c = c_;
   const auto t0 = std::chrono::steady_clock::now();
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS, NH_ERR_STATE,
-           "Lanczos: build the Hamiltonian first");
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
+           NH_ERR_STATE, "Lanczos: build the Hamiltonian first");
   NH_CHECK(o->max_iter > 2 && o->conv_tolerance > 0, NH_ERR_ARG,
            "Lanczos: bad max_iter / tolerance");
   n_ = c->n;

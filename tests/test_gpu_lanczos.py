@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 import _models
+from nemo3d_amd import native
 from nemo3d_amd.hamiltonian import Hamiltonian
 from nemo3d_amd.solvers import eigensolve_lanczos
 from oracle import oracle
@@ -26,8 +27,9 @@ def distinct(w, tol=1e-8):
     return np.array(out)
 
 
+@pytest.mark.parametrize("layout", ["blocks", "sk"])
 @pytest.mark.parametrize("case", ["si10", "gaas20"])
-def test_lanczos_matches_dense(case):
+def test_lanczos_matches_dense(case, layout):
     if case == "si10":
         M = _models.si_box(cells=(3, 3, 3), strain_nm=0.01, seed=21)
     else:
@@ -36,7 +38,8 @@ def test_lanczos_matches_dense(case):
     ref = oracle.assemble(M.arrays())
     w = distinct(np.linalg.eigvalsh(ref.dense()))
     wv, wc = gap_windows(w)
-    H = Hamiltonian(M)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK if layout == "sk"
+                    else native.LAYOUT_BLOCKS)
     r = eigensolve_lanczos(H, num_vb=3, num_cb=3, erange_vb=wv, erange_cb=wc,
                            max_iter=6000, tol=1e-10, start_iter=100, skip=25,
                            return_vectors=True)
