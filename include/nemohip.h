@@ -171,6 +171,54 @@ typedef struct nh_eig_result {
 nh_status nh_eigensolve_lanczos(nh_context *ctx, const nh_lanczos_opts *opts,
                                 nh_eig_result *res);
 
+/* Block Lanczos options: the BLanczos deck block (BL_size, ...) plus the
+ * Lanczos windows. */
+typedef struct nh_blanczos_opts {
+  int32_t block_size;           /* BL_size (1..16) */
+  int32_t max_iter;             /* block steps */
+  double conv_tolerance;        /* residual estimate ||B_m s_m|| <= tol*||T|| */
+  int32_t num_eig_vb;
+  int32_t num_eig_cb;
+  double erange_vb[2];
+  double erange_cb[2];
+  int32_t conv_check_start;     /* block steps before the first check */
+  int32_t conv_check_skip;      /* block steps between checks */
+  uint64_t seed;                /* random start block seed */
+  const nh_cplx *start_block;   /* optional host block [block_size][N] */
+  int32_t want_vectors;
+} nh_blanczos_opts;
+
+/* Block Lanczos without reorthogonalisation (eigensolve_blk_lanczos,
+ * lanczos_blk.c:2145-...): block three-term recurrence with a QR of each
+ * new block, Ritz values of the block tridiagonal in the windows, Ritz
+ * vectors by a second pass, FP64 residuals.  Degenerate levels (Kramers
+ * pairs) come out as separate pairs. */
+nh_status nh_eigensolve_block_lanczos(nh_context *ctx,
+                                      const nh_blanczos_opts *opts,
+                                      nh_eig_result *res);
+
+/* PQ-TRACEMIN options (PQTracemin deck block: NumEigVal, RefEigVal,
+ * Tolerance, cgtol, maxi, cgmaxim). */
+typedef struct nh_tracemin_opts {
+  int32_t num_eig;            /* NumEigVal: eigenvalues nearest ref_eig */
+  int32_t block_size;         /* trace-minimisation block (>= num_eig), 0: 2x */
+  double ref_eig;             /* RefEigVal sigma */
+  double tolerance;           /* ||H x - lambda x|| <= tolerance * hnorm */
+  double hnorm;               /* scale of the residual test (e.g. ||H||) */
+  double cgtol;               /* inner CG relative stopping bound */
+  int32_t max_outer;          /* maxi */
+  int32_t cg_max;             /* cgmaxim */
+  uint64_t seed;
+  const nh_cplx *start_block; /* optional host block [block_size][N] */
+} nh_tracemin_opts;
+
+/* PQ-TRACEMIN on (H - sigma)^2 (qtracemin_driver.c:9-947,
+ * pqtracemin_functions.f90:182-...): eigenpairs of H nearest sigma with
+ * FP64 Ritz values and residuals.  Eigenvectors are returned when
+ * res->eigenvectors is set. */
+nh_status nh_eigensolve_tracemin(nh_context *ctx, const nh_tracemin_opts *opts,
+                                 nh_eig_result *res);
+
 /* ---- valence force field ------------------------------------------------
  * Keating VFF (calc_estrain / min_estrain, strain3d.c:157-475,1361-2245):
  * alpha, beta, a0 are [S][S] tables (Keating constants and unstrained

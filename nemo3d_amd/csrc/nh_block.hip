@@ -1,0 +1,412 @@
+// Block Lanczos (eigensolve_blk_lanczos, NEMO_3D/eigen/src/lanczos_blk.c:
+// 2145-...) on the device.
+//
+// Recurrence (single_iteration, lanczos_blk.c:392-550), block width p:
+//   W = (H Q_j)  [block H·V, FP64 or FP32 storage]
+//   W -= Q_{j-1} B_{j-1}^H
+//   A_j = Q_j^H W ;  W -= Q_j A_j
+//   W = Q_{j+1} B_j  (Cholesky QR applied twice; the reference uses a
+//                     Gram-Schmidt QR, block_Lanczos_QR, lanczos_blk.c:273-357)
+// The block tridiagonal T (diagonal blocks A_j, sub-diagonal B_j) is a
+// Hermitian band matrix of bandwidth p; its eigenvalues in the Erange
+// windows come from Sturm (LDL^H inertia) bisection on the band and their
+// residual estimates ||B_m s_m|| from band inverse iteration (the reference
+// calls zhbevx on the band, solve_block_tridiagonal_eigenvalues,
+// lanczos_blk.c:891-991).  Eigenvectors come from a second pass over the
+// stored A_j, B_j.  All vector arithmetic is FP64.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "nh_blockla.h"
+#include "nh_internal.h"
+#include "nh_vector.h"
+
+namespace nh {
+namespace {
+
+struct Pair {
+  double value;
+  int block;     // blocks used when it converged
+  int window;
+};
+
+class BlockLanczos {
+ public:
+  explicit BlockLanczos(nh_context *c) : c_(c) {}
+  ~BlockLanczos() {
+    for (void *q : mem_) hipFree(q);
+  }
+  nh_status run(const nh_blanczos_opts *o, nh_eig_result *res);
+
+ private:
+  nh_status alloc(void **q, size_t bytes) {
+    NH_HIP(hipMalloc(q, bytes));
+    mem_.push_back(*q);
+    return NH_OK;
+  }
+  nh_status gram(const double2 *X, const double2 *Y, std::vector<cd> &G);
+  nh_status gemm(const double2 *X, int qx, double2 *Y, int qy,
+                 const std::vector<cd> &M, double beta);
+  nh_status orthonormalise(double2 *W, double2 *Qout, std::vector<cd> &R);
+  nh_status step(int j, bool first_pass);
+  void analyse(const nh_blanczos_opts *o, double hnorm);
+
+  nh_context *c_;
+  std::vector<void *> mem_;
+  int p_ = 0;
+  int64_t n_ = 0;
+  double2 *Qprev_ = nullptr, *Q_ = nullptr, *W_ = nullptr;
+  double *dM_ = nullptr, *part_ = nullptr, *sum_ = nullptr;
+  int nparts_ = 0;
+  BandT T_;
+  std::vector<cd> Bnext_;
+  std::vector<Pair> conv_;
+  std::vector<double> lastRitz_[2];
+  std::vector<char> lastConv_[2];
+};
+
+nh_status BlockLanczos::gram(const double2 *X, const double2 *Y,
+                             std::vector<cd> &G) {
+  // 8x8 tiles of G = X^H Y (a larger register tile would spill)
+  const int p = p_;
+  constexpr int P = 8;
+  G.assign(p * p, 0.0);
+  std::vector<double> h(2 * P * P);
+  for (int a0 = 0; a0 < p; a0 += P)
+    for (int b0 = 0; b0 < p; b0 += P) {
+      const int qa = std::min(P, p - a0), qb = std::min(P, p - b0);
+      const int q = std::max(qa, qb);
+      hipLaunchKernelGGL((k_gram<P>), dim3(nparts_), dim3(256), 0, c_->stream,
+                         n_, q, X + (size_t)a0 * n_, n_, Y + (size_t)b0 * n_,
+                         n_, part_);
+      hipLaunchKernelGGL(k_sum_cols, dim3(4), dim3(256), 0, c_->stream,
+                         nparts_, 2 * P * P, part_, sum_);
+      NH_HIP(hipGetLastError());
+      NH_HIP(hipMemcpyAsync(h.data(), sum_, h.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, c_->stream));
+      NH_HIP(hipStreamSynchronize(c_->stream));
+      for (int a = 0; a < qa; ++a)
+        for (int b = 0; b < qb; ++b)
+          G[(a0 + a) * p + b0 + b] = cd(h[2 * (a * P + b)], h[2 * (a * P + b) + 1]);
+    }
+  return NH_OK;
+}
+
+nh_status BlockLanczos::gemm(const double2 *X, int qx, double2 *Y, int qy,
+                             const std::vector<cd> &M, double beta) {
+  std::vector<double> h(2 * qx * qy);
+  for (int k = 0; k < qx * qy; ++k) {
+    h[2 * k] = M[k].real();
+    h[2 * k + 1] = M[k].imag();
+  }
+  NH_HIP(hipMemcpyAsync(dM_, h.data(), h.size() * sizeof(double),
+                        hipMemcpyHostToDevice, c_->stream));
+  int64_t g = (n_ + 255) / 256;
+  if (g > 16384) g = 16384;
+  const int P = std::max(qx, qy) <= 4 ? 4 : std::max(qx, qy) <= 8 ? 8 : 16;
+#define GM(PP)                                                                  \
+  hipLaunchKernelGGL((k_block_gemm<PP>), dim3((int)g), dim3(256), 0, c_->stream, \
+                     n_, qx, qy, X, n_, Y, n_, dM_, beta)
+  if (P == 4) GM(4);
+  else if (P == 8) GM(8);
+  else GM(16);
+#undef GM
+  NH_HIP(hipGetLastError());
+  // the host array h is released on return: wait for the copy
+  NH_HIP(hipStreamSynchronize(c_->stream));
+  return NH_OK;
+}
+
+// Qout R = W (CholQR, twice); R upper triangular, returned.
+nh_status BlockLanczos::orthonormalise(double2 *W, double2 *Qout,
+                                       std::vector<cd> &R) {
+  const int p = p_;
+  std::vector<cd> G;
+  R.assign(p * p, 0.0);
+  for (int k = 0; k < p; ++k) R[k * p + k] = 1.0;
+  double2 *src = W;
+  for (int pass = 0; pass < 2; ++pass) {
+    NH_TRY(gram(src, src, G));
+    // tiny diagonal lift keeps Cholesky alive for a nearly dependent block
+    double tr = 0.0;
+    for (int k = 0; k < p; ++k) tr += G[k * p + k].real();
+    for (int k = 0; k < p; ++k) G[k * p + k] += 1e-28 * tr;
+    NH_CHECK(cholesky_upper(p, G), NH_ERR_NOCONV,
+             "block Lanczos: block became rank deficient");
+    NH_TRY(gemm(src, p, Qout, p, inv_upper(p, G), 0.0));
+    R = matmul(p, G, R);
+    src = Qout;
+  }
+  return NH_OK;
+}
+
+nh_status BlockLanczos::step(int j, bool first_pass) {
+  const int p = p_;
+  // W = H Q_j
+  NH_TRY(launch_hv(c_, 0.0, reinterpret_cast<const nh_cplx *>(Q_),
+                   reinterpret_cast<nh_cplx *>(W_), p, n_));
+  if (j > 0) {  // W -= Q_{j-1} B_{j-1}^H
+    const auto &B = T_.B[j - 1];
+    std::vector<cd> M(p * p);
+    for (int k = 0; k < p; ++k)
+      for (int c = 0; c < p; ++c) M[k * p + c] = -std::conj(B[c * p + k]);
+    NH_TRY(gemm(Qprev_, p, W_, p, M, 1.0));
+  }
+  std::vector<cd> A;
+  if (first_pass) {
+    NH_TRY(gram(Q_, W_, A));
+    for (int a = 0; a < p; ++a)  // Hermitian part
+      for (int b = a; b < p; ++b) {
+        const cd h = 0.5 * (A[a * p + b] + std::conj(A[b * p + a]));
+        A[a * p + b] = h;
+        A[b * p + a] = std::conj(h);
+      }
+    T_.A.push_back(A);
+  } else {
+    A = T_.A[j];
+  }
+  std::vector<cd> M(p * p);
+  for (int k = 0; k < p * p; ++k) M[k] = -A[k];
+  NH_TRY(gemm(Q_, p, W_, p, M, 1.0));
+  std::vector<cd> R;
+  // Qprev <- new block (the old Q_{j-1} is no longer needed)
+  if (first_pass) {
+    NH_TRY(orthonormalise(W_, Qprev_, R));
+    T_.B.push_back(R);
+  } else {
+    NH_TRY(gemm(W_, p, Qprev_, p, inv_upper(p, T_.B[j]), 0.0));
+  }
+  std::swap(Qprev_, Q_);  // Q_ = Q_{j+1}, Qprev_ = Q_j
+  return NH_OK;
+}
+
+void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
+  T_.m = (int)T_.A.size();
+  T_.p = p_;
+  const double tol = o->conv_tolerance * hnorm;
+  const double resolution = 1e-9 * hnorm;
+  std::vector<cd> work, x;
+  const int req[2] = {o->num_eig_vb, o->num_eig_cb};
+  const double *win[2] = {o->erange_vb, o->erange_cb};
+  const auto &Bm = T_.B.back();
+  const int p = p_, n = T_.n();
+  for (int w = 0; w < 2; ++w) {
+    lastRitz_[w].clear();
+    lastConv_[w].clear();
+    if (req[w] <= 0) continue;
+    const double lo = win[w][0], hi = win[w][1];
+    const int klo = band_count(T_, lo, work), khi = band_count(T_, hi, work);
+    const int want = req[w] + 8;
+    int k0 = klo, k1 = khi;
+    if (w == 0) k0 = std::max(klo, khi - want);
+    else k1 = std::min(khi, klo + want);
+    for (int k = k0; k < k1; ++k) {
+      double a = lo, b = hi;
+      for (int it = 0; it < 200 && b - a > 1e-13 * (1.0 + std::fabs(a)); ++it) {
+        const double mid = 0.5 * (a + b);
+        if (band_count(T_, mid, work) > k) b = mid;
+        else a = mid;
+      }
+      const double ev = 0.5 * (a + b);
+      band_eigvec(T_, ev, x);
+      // residual estimate ||B_m s_m|| (s_m: last block of the Ritz vector)
+      double r2 = 0.0;
+      for (int rr = 0; rr < p; ++rr) {
+        cd s = 0.0;
+        for (int cc = 0; cc < p; ++cc) s += Bm[rr * p + cc] * x[n - p + cc];
+        r2 += std::norm(s);
+      }
+      const bool ok = std::sqrt(r2) <= tol;
+      lastRitz_[w].push_back(ev);
+      lastConv_[w].push_back(ok);
+      if (ok) {
+        bool known = false;
+        for (const auto &cv : conv_)
+          if (std::fabs(cv.value - ev) < resolution && cv.window == w) known = true;
+        if (!known) conv_.push_back({ev, T_.m, w});
+      }
+    }
+  }
+}
+
+nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
+  nh_context *c = c_;
+  const auto t0 = std::chrono::steady_clock::now();
+  p_ = o->block_size;
+  NH_CHECK(p_ >= 1 && p_ <= kMaxP, NH_ERR_ARG, "block size must be 1..%d", kMaxP);
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
+           NH_ERR_STATE, "block Lanczos: build the Hamiltonian first");
+  n_ = c->n;
+  const size_t blk = (size_t)p_ * n_ * sizeof(double2);
+  NH_TRY(alloc((void **)&Qprev_, blk));
+  NH_TRY(alloc((void **)&Q_, blk));
+  NH_TRY(alloc((void **)&W_, blk));
+  NH_TRY(alloc((void **)&dM_, 2 * kMaxP * kMaxP * sizeof(double)));
+  nparts_ = 1024;
+  NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * kMaxP * kMaxP * sizeof(double)));
+  NH_TRY(alloc((void **)&sum_, 2 * kMaxP * kMaxP * sizeof(double)));
+
+  // start block: caller's block, or uniform(0,1) real and imaginary parts
+  // from a fixed seed (the reference fills it with zlarnv / rand(),
+  // lanczos_blk.c:2214-2229)
+  std::vector<nh_cplx> S((size_t)p_ * n_);
+  if (o->start_block) {
+    std::copy(o->start_block, o->start_block + (size_t)p_ * n_, S.begin());
+  } else {
+    uint64_t s = 0x853C49E6748FEA9Bull ^ (uint64_t)o->seed;
+    for (auto &v : S) {
+      s = s * 6364136223846793005ull + 1442695040888963407ull;
+      v.re = (double)(s >> 11) / 9007199254740992.0;
+      s = s * 6364136223846793005ull + 1442695040888963407ull;
+      v.im = (double)(s >> 11) / 9007199254740992.0;
+    }
+  }
+  NH_HIP(hipMemcpyAsync(W_, S.data(), blk, hipMemcpyHostToDevice, c->stream));
+  std::vector<cd> R0;
+  NH_TRY(orthonormalise(W_, Q_, R0));
+  // keep Q_1 for the eigenvector pass
+  double2 *Q1 = nullptr;
+  NH_TRY(alloc((void **)&Q1, blk));
+  NH_HIP(hipMemcpyAsync(Q1, Q_, blk, hipMemcpyDeviceToDevice, c->stream));
+
+  int j = 0, skip = 0, matvecs = 0;
+  bool done = false;
+  double hnorm = 1.0;
+  while (!done && j < o->max_iter) {
+    NH_TRY(step(j, true));
+    ++j;
+    ++skip;
+    matvecs += p_;
+    if (j > o->conv_check_start && skip >= o->conv_check_skip) {
+      skip = 0;
+      // ||H|| estimate from the extreme Ritz values (Gershgorin on T)
+      T_.m = j;
+      T_.p = p_;
+      double g = 0.0;
+      for (int r = 0; r < T_.n(); ++r) {
+        double s = 0.0;
+        for (int cc = std::max(0, r - p_); cc <= std::min(T_.n() - 1, r + p_); ++cc)
+          s += std::abs(cc >= r ? T_.at(r, cc) : std::conj(T_.at(cc, r)));
+        g = std::max(g, s);
+      }
+      hnorm = g;
+      analyse(o, hnorm);
+      auto wdone = [&](int w, int req) {
+        if (req <= 0) return true;
+        const auto &r = lastRitz_[w];
+        if ((int)r.size() < req) return false;
+        for (int t = 0; t < req; ++t)
+          if (!lastConv_[w][w == 0 ? r.size() - 1 - t : t]) return false;
+        return true;
+      };
+      done = wdone(0, o->num_eig_vb) && wdone(1, o->num_eig_cb);
+    }
+  }
+  if (!done) analyse(o, hnorm);
+
+  std::vector<double> pick;
+  const int req[2] = {o->num_eig_vb, o->num_eig_cb};
+  for (int w = 0; w < 2; ++w) {
+    const auto &r = lastRitz_[w];
+    for (int t = 0; t < (int)r.size() && t < req[w]; ++t) {
+      const int idx = w == 0 ? (int)r.size() - 1 - t : t;
+      if (!lastConv_[w][idx]) break;
+      pick.push_back(r[idx]);
+    }
+  }
+  std::sort(pick.begin(), pick.end());
+  const int ne = std::min((int)pick.size(), res->capacity);
+  res->n_found = ne;
+  res->iterations = j;
+  for (int e = 0; e < ne; ++e) {
+    res->eigenvalues[e] = pick[e];
+    if (res->residuals) res->residuals[e] = -1.0;
+  }
+  if (o->want_vectors && ne > 0) {
+    // eigenvectors of T (all j blocks), then a second pass accumulating
+    // V_e += Q_k s_e[k]
+    T_.m = j;
+    std::vector<std::vector<cd>> s(ne);
+    for (int e = 0; e < ne; ++e) {
+      std::vector<const std::vector<cd> *> cluster;
+      for (int f = 0; f < e; ++f)
+        if (std::fabs(pick[f] - pick[e]) <= 1e-9 * (1.0 + std::fabs(pick[e])))
+          cluster.push_back(&s[f]);
+      band_eigvec(T_, pick[e], s[e], (uint64_t)e, cluster);
+    }
+    double2 *V = nullptr;
+    NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
+    NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
+    NH_HIP(hipMemcpyAsync(Q_, Q1, blk, hipMemcpyDeviceToDevice, c->stream));
+    const int steps = j;
+    for (int k = 0; k < steps; ++k) {
+      // V[:, e] += Q_k s_e[k*p .. k*p+p)
+      for (int e0 = 0; e0 < ne; e0 += kMaxP) {
+        const int qe = std::min(kMaxP, ne - e0);
+        std::vector<cd> M(p_ * qe);
+        for (int a = 0; a < p_; ++a)
+          for (int e = 0; e < qe; ++e) M[a * qe + e] = s[e0 + e][k * p_ + a];
+        // gemm writes qe columns with leading dimension n_
+        std::vector<double> h(2 * p_ * qe);
+        for (int t = 0; t < p_ * qe; ++t) {
+          h[2 * t] = M[t].real();
+          h[2 * t + 1] = M[t].imag();
+        }
+        NH_HIP(hipMemcpyAsync(dM_, h.data(), h.size() * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+        int64_t g = (n_ + 255) / 256;
+        if (g > 16384) g = 16384;
+        hipLaunchKernelGGL((k_block_gemm<kMaxP>), dim3((int)g), dim3(256), 0,
+                           c->stream, n_, p_, qe, Q_, n_, V + (size_t)e0 * n_,
+                           n_, dM_, 1.0);
+        NH_HIP(hipGetLastError());
+        NH_HIP(hipStreamSynchronize(c->stream));
+      }
+      if (k + 1 < steps) NH_TRY(step(k, false));
+      matvecs += p_;
+    }
+    // normalise, Rayleigh quotient, FP64 residual
+    double2 *Y = W_;
+    for (int e = 0; e < ne; ++e) {
+      const nh_cplx *v = reinterpret_cast<const nh_cplx *>(V + (size_t)e * n_);
+      double nrm2;
+      NH_TRY(reduce_nrm2sq(c, n_, v, &nrm2));
+      const nh_cplx sc = {1.0 / std::sqrt(nrm2), 0.0}, z = {0.0, 0.0};
+      NH_TRY(launch_axpby(c, n_, sc, v, z, reinterpret_cast<nh_cplx *>(V + (size_t)e * n_)));
+      NH_TRY(launch_hv(c, 0.0, v, reinterpret_cast<nh_cplx *>(Y), 1, n_));
+      ++matvecs;
+      double rq[2];
+      NH_TRY(reduce_dot(c, n_, v, reinterpret_cast<nh_cplx *>(Y), rq));
+      const nh_cplx a = {-rq[0], 0.0}, b = {1.0, 0.0};
+      NH_TRY(launch_axpby(c, n_, a, v, b, reinterpret_cast<nh_cplx *>(Y)));
+      double r2;
+      NH_TRY(reduce_nrm2sq(c, n_, reinterpret_cast<nh_cplx *>(Y), &r2));
+      res->eigenvalues[e] = rq[0];
+      if (res->residuals) res->residuals[e] = std::sqrt(r2);
+      if (res->eigenvectors)
+        NH_HIP(hipMemcpy(res->eigenvectors + (size_t)e * n_, v,
+                         n_ * sizeof(nh_cplx), hipMemcpyDeviceToHost));
+    }
+  }
+  NH_HIP(hipStreamSynchronize(c->stream));
+  res->matvecs = matvecs;
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return done ? NH_OK : NH_ERR_NOCONV;
+}
+
+}  // namespace
+}  // namespace nh
+
+extern "C" nh_status nh_eigensolve_block_lanczos(nh_context *ctx,
+                                                 const nh_blanczos_opts *opts,
+                                                 nh_eig_result *res) {
+  NH_CHECK(ctx && opts && res && res->eigenvalues && res->capacity > 0,
+           NH_ERR_ARG, "nh_eigensolve_block_lanczos: bad arguments");
+  (void)hipGetLastError();
+  NH_HIP(hipSetDevice(ctx->device));
+  nh::BlockLanczos b(ctx);
+  return b.run(opts, res);
+}

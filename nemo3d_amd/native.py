@@ -68,6 +68,38 @@ class NHLanczosOpts(ctypes.Structure):
     ]
 
 
+class NHBLanczosOpts(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32),
+        ("conv_tolerance", ctypes.c_double),
+        ("num_eig_vb", ctypes.c_int32),
+        ("num_eig_cb", ctypes.c_int32),
+        ("erange_vb", ctypes.c_double * 2),
+        ("erange_cb", ctypes.c_double * 2),
+        ("conv_check_start", ctypes.c_int32),
+        ("conv_check_skip", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("start_block", ctypes.c_void_p),
+        ("want_vectors", ctypes.c_int32),
+    ]
+
+
+class NHTraceminOpts(ctypes.Structure):
+    _fields_ = [
+        ("num_eig", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("ref_eig", ctypes.c_double),
+        ("tolerance", ctypes.c_double),
+        ("hnorm", ctypes.c_double),
+        ("cgtol", ctypes.c_double),
+        ("max_outer", ctypes.c_int32),
+        ("cg_max", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("start_block", ctypes.c_void_p),
+    ]
+
+
 class NHEigResult(ctypes.Structure):
     _fields_ = [
         ("n_found", ctypes.c_int32),
@@ -105,6 +137,8 @@ SIGNATURES = {
     "nh_nrm2": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "nh_axpby": (ctypes.c_int, [P, ctypes.c_int64, NHCplx, P, NHCplx, P]),
     "nh_eigensolve_lanczos": (ctypes.c_int, [P, P, P]),
+    "nh_eigensolve_block_lanczos": (ctypes.c_int, [P, P, P]),
+    "nh_eigensolve_tracemin": (ctypes.c_int, [P, P, P]),
     "nh_vff_energy_grad": (ctypes.c_int, [P, P, P, P, P, P, P]),
     "nh_vff_relax": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_double,
                                     P, P]),

@@ -69,3 +69,87 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     if vec is not None:
         out["vectors"] = vec[:n]
     return out
+
+
+def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
+                             erange_vb=(-0.6, 0.2), erange_cb=(1.4, 2.0),
+                             max_iter=2000, tol=1e-8, start_iter=20, skip=10,
+                             seed=1, start_block=None, want_vectors=True,
+                             capacity=64, return_vectors=False):
+    """Block Lanczos (eigensolve_blk_lanczos, NEMO_3D/eigen/src/
+    lanczos_blk.c:2145-...) with BL_size = block_size."""
+    lib = native.load()
+    o = native.NHBLanczosOpts()
+    o.block_size = int(block_size)
+    o.max_iter = int(max_iter)
+    o.conv_tolerance = float(tol)
+    o.num_eig_vb = int(num_vb)
+    o.num_eig_cb = int(num_cb)
+    o.erange_vb[0], o.erange_vb[1] = map(float, erange_vb)
+    o.erange_cb[0], o.erange_cb[1] = map(float, erange_cb)
+    o.conv_check_start = int(start_iter)
+    o.conv_check_skip = int(skip)
+    o.seed = int(seed)
+    if start_block is not None:
+        sb = np.ascontiguousarray(start_block, dtype=np.complex128)
+        assert sb.shape == (block_size, H.n)
+        o.start_block = native.ptr(sb)
+    o.want_vectors = int(bool(want_vectors))
+    ev = np.zeros(capacity)
+    rs = np.full(capacity, -1.0)
+    vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
+    r = native.NHEigResult()
+    r.eigenvalues = native.ptr(ev)
+    r.residuals = native.ptr(rs)
+    r.eigenvectors = native.ptr(vec) if vec is not None else None
+    r.capacity = capacity
+    st = lib.nh_eigensolve_block_lanczos(H.ctx, ctypes.byref(o), ctypes.byref(r))
+    if st not in (native.NH_OK, 5):
+        native.check(st)
+    n = r.n_found
+    out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
+           "iterations": r.iterations, "matvecs": r.matvecs,
+           "seconds": r.seconds, "converged": st == native.NH_OK}
+    if vec is not None:
+        out["vectors"] = vec[:n]
+    return out
+
+
+def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
+                        hnorm=40.0, cgtol=1.0, max_outer=40, cg_max=100,
+                        seed=1, start_block=None, capacity=64,
+                        return_vectors=False):
+    """PQ-TRACEMIN on (H - ref_eig)^2 (qtracemin_driver,
+    NEMO_3D/nemo3d/src/base/qtracemin_driver.c:9-947)."""
+    lib = native.load()
+    o = native.NHTraceminOpts()
+    o.num_eig = int(num_eig)
+    o.block_size = int(block_size)
+    o.ref_eig = float(ref_eig)
+    o.tolerance = float(tol)
+    o.hnorm = float(hnorm)
+    o.cgtol = float(cgtol)
+    o.max_outer = int(max_outer)
+    o.cg_max = int(cg_max)
+    o.seed = int(seed)
+    if start_block is not None:
+        sb = np.ascontiguousarray(start_block, dtype=np.complex128)
+        o.start_block = native.ptr(sb)
+    ev = np.zeros(capacity)
+    rs = np.full(capacity, -1.0)
+    vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
+    r = native.NHEigResult()
+    r.eigenvalues = native.ptr(ev)
+    r.residuals = native.ptr(rs)
+    r.eigenvectors = native.ptr(vec) if vec is not None else None
+    r.capacity = capacity
+    st = lib.nh_eigensolve_tracemin(H.ctx, ctypes.byref(o), ctypes.byref(r))
+    if st not in (native.NH_OK, 5):
+        native.check(st)
+    n = r.n_found
+    out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
+           "iterations": r.iterations, "matvecs": r.matvecs,
+           "seconds": r.seconds, "converged": st == native.NH_OK}
+    if vec is not None:
+        out["vectors"] = vec[:n]
+    return out

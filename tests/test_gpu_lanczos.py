@@ -55,3 +55,51 @@ def test_lanczos_matches_dense(case, layout):
         res = np.linalg.norm(ref.hv(v) - e * v)
         assert res < 1e-6, res
     assert (r["residuals"] < 1e-6).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", ["si10", "gaas20"])
+def test_block_lanczos_matches_dense(case, precision):
+    from nemo3d_amd.solvers import eigensolve_block_lanczos
+    if case == "si10":
+        M = _models.si_box(cells=(3, 3, 3), strain_nm=0.01, seed=31)
+    else:
+        M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=32)
+    print("case", case, precision, flush=True)
+    ref = oracle.assemble(M.arrays())
+    wall = np.linalg.eigvalsh(ref.dense())
+    w = distinct(wall)
+    wv, wc = gap_windows(w)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK,
+                    precision=native.PREC_FP64 if precision == "fp64"
+                    else native.PREC_FP32)
+    r = eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
+                                 erange_vb=wv, erange_cb=wc, max_iter=800,
+                                 tol=1e-9 if precision == "fp64" else 1e-6,
+                                 start_iter=5, skip=5, return_vectors=True)
+    assert r["converged"], r
+    got = np.sort(r["eigenvalues"])
+    want = np.sort(np.concatenate([wall[(wall > wv[0]) & (wall <= wv[1])][-4:],
+                                   wall[(wall > wc[0]) & (wall <= wc[1])][:4]]))
+    tol_e = 1e-9 if precision == "fp64" else 1e-5
+    assert np.abs(got - want).max() < tol_e, (got, want)
+    for e, v in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(v) - e * v) < (1e-6 if precision == "fp64" else 1e-4)
+
+
+def test_tracemin_matches_dense():
+    """PQ-TRACEMIN: the eigenvalues nearest sigma of a small strained Si box."""
+    from nemo3d_amd.solvers import eigensolve_tracemin
+    M = _models.si_box(cells=(2, 2, 2), strain_nm=0.01, seed=41)
+    ref = oracle.assemble(M.arrays())
+    w = np.linalg.eigvalsh(ref.dense())
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    sigma = 1.0
+    r = eigensolve_tracemin(H, num_eig=4, block_size=8, ref_eig=sigma,
+                            tol=1e-9, hnorm=40.0, cgtol=1.0, max_outer=300,
+                            cg_max=60, return_vectors=True)
+    assert r["converged"], r
+    want = np.sort(w[np.argsort(np.abs(w - sigma))[:4]])
+    assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
+    for e, v in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(v) - e * v) < 1e-7
