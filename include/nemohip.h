@@ -237,6 +237,40 @@ nh_status nh_vff_relax(nh_context *ctx, const double *alpha,
                        const double *beta, const double *a0, int max_iter,
                        double gtol, double *disp_out, double *info);
 
+/* ---- learned warm start: 3-D CNN on the cation grid ----------------------
+ * L conv layers 3x3x3 (width 32, ReLU) + 1x1x1 head; FP32 master weights,
+ * BF16 weights in the products, FP32 accumulation.  Grids are
+ * [gx][gy][gz][channels] (channels last).  No counterpart in the
+ * reference: it supplies the start block that the reference fills with
+ * random numbers (lanczos_blk.c:2214-2229, qtracemin_driver.c:112-116). */
+typedef struct nh_cnn nh_cnn;
+nh_status nh_cnn_create(nh_context *ctx, int c_in, int width, int layers,
+                        int c_out, uint64_t seed, nh_cnn **out);
+void nh_cnn_destroy(nh_cnn *cnn);
+int64_t nh_cnn_num_params(const nh_cnn *cnn);
+nh_status nh_cnn_get_params(nh_cnn *cnn, float *host);
+nh_status nh_cnn_set_params(nh_cnn *cnn, const float *host);
+/* out[gx*gy*gz][c_out] (host) */
+nh_status nh_cnn_forward(nh_cnn *cnn, const float *feat, int gx, int gy,
+                         int gz, float *out);
+/* gradient of sum(out * dout) with respect to the parameters (host) */
+nh_status nh_cnn_backward(nh_cnn *cnn, const float *feat, int gx, int gy,
+                          int gz, const float *dout, float *grad);
+/* One Adam step on one structure of the context: targets [n_states][N]
+ * (host, eigenvectors of the plain solver), cluster[n_states] groups
+ * degenerate targets; atom_point/atom_slot map atoms to grid points and
+ * to the cation (0) / anion (1) half of the point's outputs. */
+nh_status nh_cnn_train_step(nh_cnn *cnn, const float *feat, int gx, int gy,
+                            int gz, const int32_t *atom_point,
+                            const int32_t *atom_slot, const nh_cplx *targets,
+                            int n_states, const int32_t *cluster, double lr,
+                            double *loss);
+/* Start block [n_states][N] predicted for the context's structure. */
+nh_status nh_cnn_start_block(nh_cnn *cnn, const float *feat, int gx, int gy,
+                             int gz, const int32_t *atom_point,
+                             const int32_t *atom_slot, int n_states,
+                             nh_cplx *block, int ptr_kind);
+
 const char *nh_last_error(void);
 
 #ifdef __cplusplus

@@ -142,6 +142,22 @@ SIGNATURES = {
     "nh_vff_energy_grad": (ctypes.c_int, [P, P, P, P, P, P, P]),
     "nh_vff_relax": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_double,
                                     P, P]),
+    "nh_cnn_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_uint64, P]),
+    "nh_cnn_destroy": (None, [P]),
+    "nh_cnn_num_params": (ctypes.c_int64, [P]),
+    "nh_cnn_get_params": (ctypes.c_int, [P, P]),
+    "nh_cnn_set_params": (ctypes.c_int, [P, P]),
+    "nh_cnn_forward": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, P]),
+    "nh_cnn_backward": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, P, P]),
+    "nh_cnn_train_step": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, P, P, P, ctypes.c_int,
+                                         P, ctypes.c_double, P]),
+    "nh_cnn_start_block": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, P, P, ctypes.c_int, P,
+                                          ctypes.c_int]),
     "nh_last_error": (ctypes.c_char_p, []),
 }
 

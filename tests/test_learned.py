@@ -1,0 +1,87 @@
+"""Learned warm start: CPU oracle of the network, features, and the device
+network against the oracle."""
+import numpy as np
+import pytest
+
+from nemo3d_amd import learned, materials, model, structure
+from oracle import cnn_oracle
+
+SHAPE = dict(c_in=5, width=32, layers=2, c_out=6)
+
+
+def tiny():
+    rng = np.random.default_rng(3)
+    offs, n = cnn_oracle.layout(**SHAPE)
+    params = rng.normal(0, 0.2, n)
+    dims = (3, 4, 3)
+    feats = rng.normal(size=(np.prod(dims), SHAPE["c_in"]))
+    dout = rng.normal(size=(np.prod(dims), SHAPE["c_out"]))
+    return params, feats, dims, dout
+
+
+def test_oracle_backward_matches_finite_differences():
+    params, feats, dims, dout = tiny()
+    g = cnn_oracle.backward(params, feats, dims, dout=dout, round_bf16=False, **SHAPE)
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for k in rng.choice(params.size, 40, replace=False):
+        pp, pm = params.copy(), params.copy()
+        pp[k] += h
+        pm[k] -= h
+        fp = (cnn_oracle.forward(pp, feats, dims, round_bf16=False, **SHAPE) * dout).sum()
+        fm = (cnn_oracle.forward(pm, feats, dims, round_bf16=False, **SHAPE) * dout).sum()
+        fd = (fp - fm) / (2 * h)
+        assert abs(g[k] - fd) <= 1e-6 * (1 + abs(fd)), (k, g[k], fd)
+
+
+def test_bf16_rounding():
+    # 1 + 2^-8 and 1 + 3*2^-8 are ties: nearest even is 1.0 and 1 + 2^-6
+    x = np.array([1.0, 1.00390625, 1.01171875, -3.14159265, 65504.0], dtype=np.float32)
+    y = cnn_oracle.bf16(x)
+    assert y[0] == 1.0 and y[1] == 1.0 and y[2] == 1.015625
+    assert abs(y[3] + 3.140625) < 1e-12
+
+
+def test_local_strain_hydrostatic():
+    a = 0.56532
+    s = structure.binary_box((3, 3, 3), "Ga", "As", (a, a, a))
+    s.disp = 0.02 * s.positions(strained=False)  # uniform 2 % expansion
+    e = learned.local_strain(s)
+    cat = s.is_cation.astype(bool)
+    assert np.allclose(e[cat][:, :3], 0.02, atol=1e-12)
+    assert np.allclose(e[cat][:, 3:], 0.0, atol=1e-12)
+
+
+def test_grid_map_pairs_anion_with_cation():
+    a = 0.56532
+    s = structure.binary_box((2, 2, 2), "Ga", "As", (a, a, a))
+    g = learned.GridMap(s)
+    # each point holds at most one cation and one anion; the anion of a
+    # point is the cation's neighbour along -(1,1,1) a/4
+    key = g.point * 2 + g.slot
+    assert np.unique(key).size == s.n_atoms
+    p = s.positions()
+    for i in np.nonzero(s.is_cation)[0][:20]:
+        j = np.nonzero((g.point == g.point[i]) & (g.slot == 1))[0]
+        assert j.size == 1
+        assert np.allclose(p[i] - p[j[0]], 0.25 * a, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_device_network_matches_oracle():
+    import _models
+    from nemo3d_amd.hamiltonian import Hamiltonian
+    M = _models.si_box(cells=(2, 2, 2))
+    H = Hamiltonian(M, layout=None)
+    params, feats, dims, dout = tiny()
+    net = learned.CNN(H, SHAPE["c_in"], SHAPE["width"], SHAPE["layers"],
+                      SHAPE["c_out"], seed=1)
+    net.set_params(params.astype(np.float32))
+    p32 = params.astype(np.float32).astype(np.float64)
+    out = net.forward(feats.astype(np.float32), dims)
+    ref = cnn_oracle.forward(p32, feats.astype(np.float32), dims, **SHAPE)
+    assert np.abs(out - ref).max() <= 1e-5 * np.abs(ref).max()
+    g = net.grad(feats.astype(np.float32), dims, dout.astype(np.float32))
+    gref = cnn_oracle.backward(p32, feats.astype(np.float32), dims,
+                               dout=dout.astype(np.float32), **SHAPE)
+    assert np.abs(g - gref).max() <= 1e-4 * np.abs(gref).max()
