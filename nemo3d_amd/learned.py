@@ -47,7 +47,8 @@ class GridMap:
 
 def local_strain(structure):
     """Strain tensor at each cation from its bonds: R = (1 + e) R0,
-    least squares over the present bonds, symmetrised.  Returns (A, 6):
+    least squares over the present bonds (directions no bond constrains are
+    left unstrained), symmetrised.  Returns (A, 6):
     exx, eyy, ezz, exy, exz, eyz (zero for anions)."""
     s = structure
     A = s.n_atoms
@@ -61,8 +62,9 @@ def local_strain(structure):
     R0 = np.where(ok[..., None], R0, 0.0)
     M = np.einsum("bni,bnj->bij", R, R0)
     G = np.einsum("bni,bnj->bij", R0, R0)
-    G = G + 1e-12 * np.eye(3)
-    F = M @ np.linalg.inv(G)
+    # pseudo-inverse: directions without bonds (surface atoms) get no strain
+    Gp = np.linalg.pinv(G, rcond=1e-9, hermitian=True)
+    F = M @ Gp + (np.eye(3) - G @ Gp)
     e = 0.5 * (F + np.swapaxes(F, 1, 2)) - np.eye(3)
     out[cat] = np.stack([e[:, 0, 0], e[:, 1, 1], e[:, 2, 2], e[:, 0, 1],
                          e[:, 0, 2], e[:, 1, 2]], axis=1)

@@ -47,9 +47,10 @@ def test_local_strain_hydrostatic():
     s = structure.binary_box((3, 3, 3), "Ga", "As", (a, a, a))
     s.disp = 0.02 * s.positions(strained=False)  # uniform 2 % expansion
     e = learned.local_strain(s)
-    cat = s.is_cation.astype(bool)
-    assert np.allclose(e[cat][:, :3], 0.02, atol=1e-12)
-    assert np.allclose(e[cat][:, 3:], 0.0, atol=1e-12)
+    bulk = s.is_cation.astype(bool) & ~s.surface_mask()
+    assert np.allclose(e[bulk][:, :3], 0.02, atol=1e-12)
+    assert np.allclose(e[bulk][:, 3:], 0.0, atol=1e-12)
+    assert np.abs(e).max() <= 0.02 + 1e-12
 
 
 def test_grid_map_pairs_anion_with_cation():
