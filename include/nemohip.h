@@ -140,8 +140,10 @@ typedef struct nh_lanczos_opts {
   int32_t cyclic_len;           /* CyclicString length */
   const double *cyclic_string;  /* CyclicString values */
   const nh_cplx *start_vector;  /* start == 2: host vector of length N */
-  double resid_tol;             /* relative residual target of the final
-                                   FP64 refinement (0: no refinement) */
+  int32_t stop_rule;            /* 0: stop once num_eig converged values lie
+                                   in each window (lanczos.c:1231-1232);
+                                   1: stop once the num_eig Ritz values
+                                   nearest the gap are all converged */
   int32_t want_vectors;         /* compute eigenvectors (second pass) */
 } nh_lanczos_opts;
 

@@ -299,7 +299,7 @@ class LanczosRun {
   nh_status iterate(int k, bool with_reductions);
   nh_status collect_timing();
   nh_status analyse(int m, const nh_lanczos_opts *o);
-  bool window_done(int w, int req) const;
+  bool window_done(int w, int req, int rule) const;
 
   nh_context *c_;
   std::vector<void *> allocs_;
@@ -403,8 +403,15 @@ nh_status LanczosRun::iterate(int k, bool red) {
   return NH_OK;
 }
 
-bool LanczosRun::window_done(int w, int req) const {
+bool LanczosRun::window_done(int w, int req, int rule) const {
   if (req <= 0) return true;
+  if (rule == 0) {
+    // reference rule (lanczos.c:1231-1232): enough converged values found
+    // anywhere in the window
+    int cnt = 0;
+    for (const auto &x : conv_) cnt += x.window == w;
+    return cnt >= req;
+  }
   // the `req` Ritz values nearest the gap (top of the vb window, bottom of
   // the cb window) must all be converged
   const auto &r = lastRitz_[w];
@@ -527,7 +534,8 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     if (o->conv_check_start < m && o->conv_check_skip <= skip) {
       skip = 0;
       NH_TRY(analyse(m, o));
-      done = window_done(0, o->num_eig_vb) && window_done(1, o->num_eig_cb);
+      done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+             window_done(1, o->num_eig_cb, o->stop_rule);
     }
   }
   if (!done) NH_TRY(analyse(m, o));
@@ -541,6 +549,17 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   std::vector<Converged> pick;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};
   for (int w = 0; w < 2; ++w) {
+    if (o->stop_rule == 0) {
+      // every converged value of the window, nearest the gap first
+      std::vector<Converged> in;
+      for (const auto &x : conv_)
+        if (x.window == w) in.push_back(x);
+      std::sort(in.begin(), in.end(), [w](const Converged &a, const Converged &b) {
+        return w == 0 ? a.value > b.value : a.value < b.value;
+      });
+      for (int t = 0; t < (int)in.size() && t < req[w]; ++t) pick.push_back(in[t]);
+      continue;
+    }
     const auto &r = lastRitz_[w];
     const auto &cv = lastConv_[w];
     int got = 0;

@@ -63,7 +63,7 @@ class NHLanczosOpts(ctypes.Structure):
         ("cyclic_len", ctypes.c_int32),
         ("cyclic_string", ctypes.c_void_p),
         ("start_vector", ctypes.c_void_p),
-        ("resid_tol", ctypes.c_double),
+        ("stop_rule", ctypes.c_int32),
         ("want_vectors", ctypes.c_int32),
     ]
 

@@ -21,7 +21,11 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                        start_iter=200, skip=50, start="CyclicRandom",
                        cyclic=DEFAULT_CYCLIC, start_vector=None,
                        want_vectors=True, capacity=64, return_vectors=False,
-                       return_tridiag=False):
+                       return_tridiag=False, stop_rule="window"):
+    """stop_rule "window": stop once num_vb / num_cb converged values lie in
+    the windows (the reference's rule, lanczos.c:1231-1232); "nearest":
+    stop once the num_vb / num_cb Ritz values nearest the gap are all
+    converged."""
     lib = native.load()
     o = native.NHLanczosOpts()
     o.max_iter = int(max_iter)
@@ -39,7 +43,7 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     if start_vector is not None:
         sv = np.ascontiguousarray(start_vector, dtype=np.complex128)
         o.start_vector = native.ptr(sv)
-    o.resid_tol = 0.0
+    o.stop_rule = {"window": 0, "nearest": 1}[stop_rule]
     o.want_vectors = int(bool(want_vectors))
     ev = np.zeros(capacity)
     rs = np.full(capacity, -1.0)

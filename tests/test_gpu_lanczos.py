@@ -42,7 +42,7 @@ def test_lanczos_matches_dense(case, layout):
                     else native.LAYOUT_BLOCKS)
     r = eigensolve_lanczos(H, num_vb=3, num_cb=3, erange_vb=wv, erange_cb=wc,
                            max_iter=6000, tol=1e-10, start_iter=100, skip=25,
-                           return_vectors=True)
+                           return_vectors=True, stop_rule="nearest")
     assert r["converged"]
     got = np.sort(r["eigenvalues"])
     top_vb = w[(w > wv[0]) & (w <= wv[1])][-3:]
@@ -54,6 +54,27 @@ def test_lanczos_matches_dense(case, layout):
     for e, v in zip(r["eigenvalues"], r["vectors"]):
         res = np.linalg.norm(ref.hv(v) - e * v)
         assert res < 1e-6, res
+    assert (r["residuals"] < 1e-6).all()
+
+
+def test_lanczos_window_rule():
+    """Reference stop rule: stop once the windows hold the requested number
+    of converged values; windows bracketing exactly the wanted states."""
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=23)
+    ref = oracle.assemble(M.arrays())
+    w = distinct(np.linalg.eigvalsh(ref.dense()))
+    wv, wc = gap_windows(w)
+    vb = w[(w > wv[0]) & (w <= wv[1])][-2:]
+    cb = w[(w > wc[0]) & (w <= wc[1])][:2]
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    r = eigensolve_lanczos(H, num_vb=2, num_cb=2,
+                           erange_vb=(0.5 * (vb[0] + w[w < vb[0]][-1]), vb[1] + 1e-3),
+                           erange_cb=(cb[0] - 1e-3, 0.5 * (cb[1] + w[w > cb[1]][0])),
+                           max_iter=6000, tol=1e-10, start_iter=100, skip=25,
+                           stop_rule="window")
+    assert r["converged"]
+    got = np.sort(r["eigenvalues"])
+    assert np.abs(got - np.sort(np.concatenate([vb, cb]))).max() < 1e-9
     assert (r["residuals"] < 1e-6).all()
 
 
