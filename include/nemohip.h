@@ -273,6 +273,29 @@ nh_status nh_cnn_start_block(nh_cnn *cnn, const float *feat, int gx, int gy,
                              const int32_t *atom_slot, int n_states,
                              nh_cplx *block, int ptr_kind);
 
+/* ---- operator callbacks with the reference signatures -------------------
+ * For the reference's unchanged solver drivers.  Host vectors; the context
+ * is bound with nh_shim_bind (argList[0] / `d` are not read).  Like the
+ * functions they replace these abort with a message on error. */
+nh_status nh_shim_bind(nh_context *ctx);
+/* The wave-function table (d->WF: energies and vectors [nwf][N], host) that
+ * matvect_mult's spectral projection indexes through eval_scoperti. */
+nh_status nh_shim_set_wavefunctions(int nwf, const double *energy,
+                                    const nh_cplx *psi);
+/* matvect_mult (h_cvectr_mult.c:205-258): argList[1] = &shift,
+ * argList[2] = &Nproj, argList[3] = &eval_scoperti when Nproj > 0;
+ * y = (H - shift) x - sum_j E_j <psi_j|x> psi_j. */
+void nh_matvect_mult(const int **argList, nh_cplx *y, const nh_cplx *x);
+/* matmat_mult (h_cmat_mult.c:21-83): argList[1] = &shift,
+ * argList[2] = &num_col; x, y are nml_dcmatrix row-pointer arrays (one row
+ * per vector).  Nproj is not applied, as in the reference. */
+void nh_matmat_mult(const int **argList, nh_cplx **y, nh_cplx *const *x);
+/* Hmult_spds_complete (h_cvectr_mult.c:262) and Hmatmult_spds_complete
+ * (h_cmat_mult.c:85); `d` is ignored. */
+void nh_Hmult_spds_complete(nh_cplx *y, void *d, double shift, nh_cplx *x);
+void nh_Hmatmult_spds_complete(nh_cplx **Y, void *d, double shift,
+                               nh_cplx **X, int num_col);
+
 const char *nh_last_error(void);
 
 #ifdef __cplusplus

@@ -25,7 +25,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-I" + os.path.join(HERE, "..", "include")]
 
 SOURCES = ["nh_api.hip", "nh_assemble.hip", "nh_hv.hip", "nh_vector.hip",
-           "nh_lanczos.hip", "nh_vff.hip", "nh_block.hip", "nh_tracemin.hip", "nh_cnn.hip"]
+           "nh_lanczos.hip", "nh_vff.hip", "nh_block.hip", "nh_tracemin.hip", "nh_cnn.hip",
+           "nh_shim.hip"]
 
 
 def _compile(src):

@@ -158,6 +158,13 @@ SIGNATURES = {
     "nh_cnn_start_block": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, P, P, ctypes.c_int, P,
                                           ctypes.c_int]),
+    "nh_shim_bind": (ctypes.c_int, [P]),
+    "nh_shim_set_wavefunctions": (ctypes.c_int, [ctypes.c_int, P, P]),
+    "nh_matvect_mult": (None, [P, P, P]),
+    "nh_matmat_mult": (None, [P, P, P]),
+    "nh_Hmult_spds_complete": (None, [P, P, ctypes.c_double, P]),
+    "nh_Hmatmult_spds_complete": (None, [P, P, ctypes.c_double, P,
+                                         ctypes.c_int]),
     "nh_last_error": (ctypes.c_char_p, []),
 }
 
