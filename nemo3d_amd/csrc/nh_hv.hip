@@ -71,6 +71,7 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
            NH_ERR_STATE, "H·v: Hamiltonian not built in a supported layout");
   NH_CHECK(ncol >= 1 && ncol <= 65535 && ld >= c->n, NH_ERR_ARG,
            "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
+  if (hV_mfma_supported(c, ncol)) return launch_hV_mfma(c, shift, x, y, ld);
   if (c->layout == NH_LAYOUT_SK) {
     const int nbc = (c->n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
     const int nba = (c->n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;

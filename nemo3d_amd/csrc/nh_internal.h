@@ -97,6 +97,10 @@ namespace nh {
 nh_status assemble(nh_context *c, int layout, int precision);
 nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x,
                     nh_cplx *y, int ncol, int64_t ld);
+// block H·V of 8 vectors on the matrix cores (FP32 builds; nh_hv_mfma.hip)
+bool hV_mfma_supported(const nh_context *c, int ncol);
+nh_status launch_hV_mfma(nh_context *c, double shift, const nh_cplx *x,
+                         nh_cplx *y, int64_t ld);
 nh_status reduce_dot(nh_context *c, int64_t n, const nh_cplx *x,
                      const nh_cplx *y, double *out2 /* host re,im */);
 nh_status reduce_nrm2sq(nh_context *c, int64_t n, const nh_cplx *x,
