@@ -258,15 +258,20 @@ nh_status nh_cnn_forward(nh_cnn *cnn, const float *feat, int gx, int gy,
 /* gradient of sum(out * dout) with respect to the parameters (host) */
 nh_status nh_cnn_backward(nh_cnn *cnn, const float *feat, int gx, int gy,
                           int gz, const float *dout, float *grad);
-/* One Adam step on one structure of the context: targets [n_states][N]
- * (host, eigenvectors of the plain solver), cluster[n_states] groups
- * degenerate targets; atom_point/atom_slot map atoms to grid points and
- * to the cation (0) / anion (1) half of the point's outputs. */
+/* One Adam step on one structure of the context.  targets [n_targets][N]
+ * (host, n_targets <= 16): eigenvectors of the plain solver, with their
+ * Kramers partners; cluster[n_targets] labels each target's degenerate
+ * subspace and out_cluster[n_states] (n_states <= 8) the subspace each
+ * predicted column should span.  Loss = mean over columns of
+ * 1 - |projection onto the subspace|^2 / |column|^2 (phase- and
+ * degeneracy-invariant).  lr = 0: no update.  grad (host [n_params], or
+ * NULL) receives the parameter gradient. */
 nh_status nh_cnn_train_step(nh_cnn *cnn, const float *feat, int gx, int gy,
                             int gz, const int32_t *atom_point,
                             const int32_t *atom_slot, const nh_cplx *targets,
-                            int n_states, const int32_t *cluster, double lr,
-                            double *loss);
+                            int n_targets, const int32_t *cluster,
+                            int n_states, const int32_t *out_cluster,
+                            double lr, double *loss, float *grad);
 /* Start block [n_states][N] predicted for the context's structure. */
 nh_status nh_cnn_start_block(nh_cnn *cnn, const float *feat, int gx, int gy,
                              int gz, const int32_t *atom_point,

@@ -587,9 +587,12 @@ nh_status nh_cnn_start_block(nh_cnn *c, const float *feat, int gx, int gy,
 
 nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz,
                             const int32_t *atom_point, const int32_t *atom_slot,
-                            const nh_cplx *targets, int n_states,
-                            const int32_t *cluster, double lr, double *loss) {
-  NH_CHECK(c && feat && atom_point && atom_slot && targets && cluster && loss &&
+                            const nh_cplx *targets, int n_targets,
+                            const int32_t *cluster, int n_states,
+                            const int32_t *out_cluster, double lr, double *loss,
+                            float *grad) {
+  NH_CHECK(c && feat && atom_point && atom_slot && targets && cluster &&
+               out_cluster && loss && n_targets >= 1 && n_targets <= 16 &&
                n_states >= 1 && n_states <= 8 &&
                n_states * 2 * c->ctx->nb * 2 <= c->c_out,
            NH_ERR_ARG, "nh_cnn_train_step: bad arguments");
@@ -599,7 +602,8 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
   const int ne = n_states;
   NH_TRY(upload_feats(c, feat, P));
   NH_TRY(forward(c, gx, gy, gz));
-  // device scratch: point/slot maps, Y (pred), V (targets), G (grad)
+  // device scratch: point/slot maps, Y (prediction, 8 columns), V (targets,
+  // 16 columns), G (gradient, 8 columns)
   std::vector<void *> mem;
   auto mk = [&](void **p, size_t bytes) -> nh_status {
     NH_HIP(hipMalloc(p, bytes));
@@ -616,7 +620,7 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
   if (st == NH_OK) st = mk((void **)&dp, A * sizeof(int32_t));
   if (st == NH_OK) st = mk((void **)&ds, A * sizeof(int32_t));
   if (st == NH_OK) st = mk((void **)&Y, (size_t)8 * N * sizeof(double2));
-  if (st == NH_OK) st = mk((void **)&V, (size_t)8 * N * sizeof(double2));
+  if (st == NH_OK) st = mk((void **)&V, (size_t)16 * N * sizeof(double2));
   if (st == NH_OK) st = mk((void **)&G, (size_t)8 * N * sizeof(double2));
   if (st == NH_OK) st = mk((void **)&part, (size_t)512 * 2 * 64 * sizeof(double));
   if (st == NH_OK) st = mk((void **)&sum, 2 * 64 * sizeof(double));
@@ -629,15 +633,22 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
   hipMemcpyAsync(dp, atom_point, A * sizeof(int32_t), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice, s);
   hipMemsetAsync(Y, 0, (size_t)8 * N * sizeof(double2), s);
-  hipMemsetAsync(V, 0, (size_t)8 * N * sizeof(double2), s);
-  hipMemcpyAsync(V, targets, (size_t)ne * N * sizeof(double2), hipMemcpyHostToDevice, s);
+  hipMemsetAsync(V, 0, (size_t)16 * N * sizeof(double2), s);
+  hipMemcpyAsync(V, targets, (size_t)n_targets * N * sizeof(double2),
+                 hipMemcpyHostToDevice, s);
   hipLaunchKernelGGL(k_map_out, dim3(g1(A * x->nb)), dim3(256), 0, s, A, x->nb,
                      ne, c->c_out, dp, ds, c->out, Y, N);
-  // C = V^H Y (8x8) and ||y_e||^2
-  std::vector<double> h(2 * 64), hb(2 * 64);
-  hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, V, N, Y, N, part);
-  hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(128), 0, s, 512, 128, part, sum);
-  hipMemcpyAsync(h.data(), sum, 128 * sizeof(double), hipMemcpyDeviceToHost, s);
+  // C = V^H Y (16 x 8, two 8x8 halves) and the Gram Y^H Y
+  std::vector<double> h(2 * 128), hb(2 * 64);
+  for (int half = 0; half < 2; ++half) {
+    hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8,
+                       V + (int64_t)half * 8 * N, N, Y, N, part);
+    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(128), 0, s, 512, 128, part, sum);
+    hipMemcpyAsync(h.data() + half * 128, sum, 128 * sizeof(double),
+                   hipMemcpyDeviceToHost, s);
+    // the next k_sum_cols overwrites `sum`: wait for the copy
+    hipStreamSynchronize(s);
+  }
   hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, Y, N, Y, N, part);
   hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(128), 0, s, 512, 128, part, sum);
   hipMemcpyAsync(hb.data(), sum, 128 * sizeof(double), hipMemcpyDeviceToHost, s);
@@ -646,32 +657,38 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
     cleanup();
     NH_CHECK(false, NH_ERR_HIP, "train step: %s", hipGetErrorString(e));
   }
-  // loss and dL/dy* = 2(-P_e y_e / b_e + a_e y_e / b_e^2) / ne
-  std::vector<cd> M(64, 0.0), D(64, 0.0);
+  // C[f][e] = <v_f, y_e>
+  auto C = [&](int f, int ee) {
+    const double *q = h.data() + (f / 8) * 128 + 2 * ((f % 8) * 8 + ee);
+    return cd(q[0], q[1]);
+  };
+  // L = mean_e (1 - a_e / b_e), a_e = sum over the targets of e's cluster of
+  // |C[f][e]|^2, b_e = ||y_e||^2;  G_e = dL/dRe(y_e) + i dL/dIm(y_e)
+  //     = 2 (-sum_f v_f C[f][e] / b_e + a_e y_e / b_e^2) / ne
+  std::vector<cd> M(128, 0.0), D(64, 0.0);
   double L = 0.0;
   for (int ee = 0; ee < ne; ++ee) {
     const double b = std::max(hb[2 * (ee * 8 + ee)], 1e-300);
     double a = 0.0;
-    for (int f = 0; f < ne; ++f)
-      if (cluster[f] == cluster[ee])
-        a += h[2 * (f * 8 + ee)] * h[2 * (f * 8 + ee)] +
-             h[2 * (f * 8 + ee) + 1] * h[2 * (f * 8 + ee) + 1];
+    for (int f = 0; f < n_targets; ++f)
+      if (cluster[f] == out_cluster[ee]) a += std::norm(C(f, ee));
     L += (1.0 - a / b) / ne;
-    for (int f = 0; f < ne; ++f)
-      if (cluster[f] == cluster[ee])
-        M[f * 8 + ee] = -2.0 * cd(h[2 * (f * 8 + ee)], h[2 * (f * 8 + ee) + 1]) / (b * ne);
+    for (int f = 0; f < n_targets; ++f)
+      if (cluster[f] == out_cluster[ee]) M[f * 8 + ee] = -2.0 * C(f, ee) / (b * ne);
     D[ee * 8 + ee] = 2.0 * a / (b * b * ne);
   }
   *loss = L;
   std::vector<double> hm(128);
-  for (int k = 0; k < 64; ++k) {
-    hm[2 * k] = M[k].real();
-    hm[2 * k + 1] = M[k].imag();
+  for (int half = 0; half < 2; ++half) {
+    for (int k = 0; k < 64; ++k) {
+      hm[2 * k] = M[half * 64 + k].real();
+      hm[2 * k + 1] = M[half * 64 + k].imag();
+    }
+    hipMemcpyAsync(dM, hm.data(), 128 * sizeof(double), hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL((k_block_gemm<8>), dim3(g1(N)), dim3(256), 0, s, N, 8, 8,
+                       V + (int64_t)half * 8 * N, N, G, N, dM, half ? 1.0 : 0.0);
+    hipStreamSynchronize(s);  // hm is rewritten next
   }
-  hipMemcpyAsync(dM, hm.data(), 128 * sizeof(double), hipMemcpyHostToDevice, s);
-  hipLaunchKernelGGL((k_block_gemm<8>), dim3(g1(N)), dim3(256), 0, s, N, 8, 8, V,
-                     N, G, N, dM, 0.0);
-  hipStreamSynchronize(s);
   for (int k = 0; k < 64; ++k) {
     hm[2 * k] = D[k].real();
     hm[2 * k + 1] = 0.0;
@@ -693,6 +710,11 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
       cleanup();
       return bs;
     }
+    if (grad)
+      e = hipMemcpyAsync(grad, c->grad, c->n_params * sizeof(float),
+                         hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess && lr > 0.0) {
     c->step += 1;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)c->step);
@@ -701,8 +723,8 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
                        c->n_params, c->master, c->grad, c->m1, c->m2, (float)lr,
                        b1, b2, c1, c2, 1e-8f);
     sync_bf16(c);
-    e = hipStreamSynchronize(s);
   }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   cleanup();
   NH_CHECK(e == hipSuccess, NH_ERR_HIP, "train step: %s", hipGetErrorString(e));
   return NH_OK;

@@ -104,6 +104,18 @@ def features(tbmodel, gmap, dot_mask=None):
     return f
 
 
+def kramers_partner(psi, nb):
+    """Time-reversed state (spin up u, spin down d per atom) -> (-d*, u*):
+    degenerate with psi and orthogonal to it for the spin-orbit
+    Hamiltonian (20-band model)."""
+    assert nb == 20
+    v = np.asarray(psi).reshape(-1, 2, 10)
+    out = np.empty_like(v)
+    out[:, 0] = -np.conj(v[:, 1])
+    out[:, 1] = np.conj(v[:, 0])
+    return out.reshape(np.shape(psi))
+
+
 class CNN:
     """Handle of the device network (libnemohip nh_cnn_*)."""
 
@@ -144,6 +156,39 @@ class CNN:
             self.h, native.ptr(np.ascontiguousarray(feats, dtype=np.float32)),
             dims[0], dims[1], dims[2], native.ptr(out)))
         return out
+
+    def start_block(self, feats, gmap, n_states=N_STATES, device=False):
+        """Predicted start block (n_states, N) for the owner's structure; a
+        native.DeviceArray when device=True."""
+        n = self.owner.n
+        if device:
+            blk = native.DeviceArray(self.owner, (n_states, n))
+            kind = native.PTR_DEVICE
+        else:
+            blk = np.zeros((n_states, n), dtype=np.complex128)
+            kind = native.PTR_HOST
+        native.check(self.lib.nh_cnn_start_block(
+            self.h, native.ptr(np.ascontiguousarray(feats, dtype=np.float32)),
+            gmap.dims[0], gmap.dims[1], gmap.dims[2], native.ptr(gmap.point),
+            native.ptr(gmap.slot), n_states, native.ptr(blk), kind))
+        return blk
+
+    def train_step(self, feats, gmap, targets, cluster, out_cluster, lr,
+                   want_grad=False):
+        """One Adam step (lr > 0) on the subspace loss; returns the loss
+        (and the parameter gradient when want_grad)."""
+        t = np.ascontiguousarray(targets, dtype=np.complex128)
+        cl = np.ascontiguousarray(cluster, dtype=np.int32)
+        oc = np.ascontiguousarray(out_cluster, dtype=np.int32)
+        loss = ctypes.c_double()
+        g = np.zeros(self.n_params, dtype=np.float32) if want_grad else None
+        native.check(self.lib.nh_cnn_train_step(
+            self.h, native.ptr(np.ascontiguousarray(feats, dtype=np.float32)),
+            gmap.dims[0], gmap.dims[1], gmap.dims[2], native.ptr(gmap.point),
+            native.ptr(gmap.slot), native.ptr(t), t.shape[0], native.ptr(cl),
+            oc.size, native.ptr(oc), float(lr), ctypes.byref(loss),
+            native.ptr(g) if g is not None else None))
+        return (loss.value, g) if want_grad else loss.value
 
     def grad(self, feats, dims, dout):
         """Parameter gradient of sum(out * dout) (backward pass)."""

@@ -154,7 +154,8 @@ SIGNATURES = {
                                        ctypes.c_int, P, P]),
     "nh_cnn_train_step": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, P, P, P, ctypes.c_int,
-                                         P, ctypes.c_double, P]),
+                                         P, ctypes.c_int, P, ctypes.c_double,
+                                         P, P]),
     "nh_cnn_start_block": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, P, P, ctypes.c_int, P,
                                           ctypes.c_int]),

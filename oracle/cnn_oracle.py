@@ -100,3 +100,46 @@ def backward(params, feats, dims, c_in, width, layers, c_out, dout,
         g[b[0]:b[1]] = dpre.sum(axis=(0, 1, 2))
         da = dx
     return g
+
+
+def map_out(out, point, slot, nb, ne):
+    """Network output (P, c_out) -> start block (ne, A*nb) complex: atom a,
+    orbital o of state e reads channels 2*((e*2 + slot[a])*nb + o) + {0, 1}
+    of grid point point[a]."""
+    out = np.asarray(out, dtype=np.float64)
+    A = point.size
+    Y = np.zeros((ne, A * nb), dtype=np.complex128)
+    o = np.arange(nb)
+    for e in range(ne):
+        ch = 2 * ((e * 2 + slot[:, None]) * nb + o[None, :])
+        Y[e] = (out[point[:, None], ch] + 1j * out[point[:, None], ch + 1]).reshape(-1)
+    return Y
+
+
+def map_grad(G, point, slot, nb, ne, n_points, c_out):
+    """Adjoint of map_out for real/imaginary channel gradients."""
+    dout = np.zeros((n_points, c_out))
+    o = np.arange(nb)
+    for e in range(ne):
+        g = G[e].reshape(-1, nb)
+        ch = 2 * ((e * 2 + slot[:, None]) * nb + o[None, :])
+        dout[point[:, None], ch] = g.real
+        dout[point[:, None], ch + 1] = g.imag
+    return dout
+
+
+def subspace_loss(Y, V, cluster, out_cluster):
+    """L = mean_e (1 - sum_{f in cluster of e} |<v_f, y_e>|^2 / |y_e|^2) and
+    G_e = dL/dRe(y_e) + i dL/dIm(y_e)."""
+    ne = Y.shape[0]
+    C = np.conj(V) @ Y.T  # C[f, e] = <v_f, y_e>
+    L = 0.0
+    G = np.zeros_like(Y)
+    for e in range(ne):
+        sel = np.asarray(cluster) == out_cluster[e]
+        b = np.vdot(Y[e], Y[e]).real
+        a = np.sum(np.abs(C[sel, e]) ** 2)
+        L += (1 - a / b) / ne
+        Py = V[sel].T @ C[sel, e]
+        G[e] = 2 * (-Py / b + a * Y[e] / b ** 2) / ne
+    return L, G

@@ -86,3 +86,90 @@ def test_device_network_matches_oracle():
     gref = cnn_oracle.backward(p32, feats.astype(np.float32), dims,
                                dout=dout.astype(np.float32), **SHAPE)
     assert np.abs(g - gref).max() <= 1e-4 * np.abs(gref).max()
+
+
+def _targets(n, rng, levels=4):
+    """Orthonormal targets: `levels` states and their Kramers partners
+    (orthonormalised together, as eigenvectors of distinct levels are)."""
+    v = rng.normal(size=(levels, n)) + 1j * rng.normal(size=(levels, n))
+    t = []
+    for e in range(levels):
+        t += [v[e], learned.kramers_partner(v[e], 20)]
+    t = np.linalg.qr(np.array(t).T)[0].T
+    return t, np.repeat(np.arange(levels), 2), np.arange(levels)
+
+
+def test_subspace_loss_gradient_and_invariances():
+    rng = np.random.default_rng(7)
+    n = 60
+    V, cl, oc = _targets(n, rng)
+    Y = rng.normal(size=(4, n)) + 1j * rng.normal(size=(4, n))
+    L, G = cnn_oracle.subspace_loss(Y, V, cl, oc)
+    h = 1e-6
+    for e, i in [(0, 3), (2, 17), (3, 59), (1, 0)]:
+        for unit in (1.0, 1j):
+            Yp, Ym = Y.copy(), Y.copy()
+            Yp[e, i] += h * unit
+            Ym[e, i] -= h * unit
+            fd = (cnn_oracle.subspace_loss(Yp, V, cl, oc)[0] -
+                  cnn_oracle.subspace_loss(Ym, V, cl, oc)[0]) / (2 * h)
+            got = G[e, i].real if unit == 1.0 else G[e, i].imag
+            assert abs(got - fd) < 1e-7, (e, i, unit, got, fd)
+    # invariant under phases of the columns and unitary mixing of a pair
+    ph = np.exp(1j * rng.uniform(0, 6.3, 4))
+    assert abs(cnn_oracle.subspace_loss(Y * ph[:, None], V, cl, oc)[0] - L) < 1e-12
+    U = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+    V2 = V.copy()
+    V2[0:2] = U @ V[0:2]
+    assert abs(cnn_oracle.subspace_loss(Y, V2, cl, oc)[0] - L) < 1e-12
+    # zero exactly on the target subspaces, one on their complement
+    Yin = np.array([V[2 * e] * 0.3 + V[2 * e + 1] * (0.2 - 1j) for e in range(4)])
+    assert abs(cnn_oracle.subspace_loss(Yin, V, cl, oc)[0]) < 1e-12
+    Yout = Y - (np.conj(V) @ Y.T).T @ V
+    assert abs(cnn_oracle.subspace_loss(Yout, V, cl, oc)[0] - 1.0) < 1e-12
+
+
+def test_kramers_partner_commutes_with_H():
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    import _models
+    from oracle import oracle
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.02, seed=3, alloy=True)
+    ref = oracle.assemble(M.arrays())
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=M.n) + 1j * rng.normal(size=M.n)
+    tx = learned.kramers_partner(x, 20)
+    assert abs(np.vdot(x, tx)) < 1e-12 * np.vdot(x, x).real
+    hx = ref.hv(x)
+    assert np.linalg.norm(ref.hv(tx) - learned.kramers_partner(hx, 20)) <= \
+        1e-13 * np.linalg.norm(hx)
+
+
+@pytest.mark.gpu
+def test_device_train_step_and_start_block_match_oracle():
+    import _models
+    from nemo3d_amd.hamiltonian import Hamiltonian
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.02, seed=8)
+    H = Hamiltonian(M, layout=None)
+    g = learned.GridMap(M.structure)
+    rng = np.random.default_rng(9)
+    ne, c_in = 4, 6
+    shape = dict(c_in=c_in, width=32, layers=2, c_out=ne * 2 * 20 * 2)
+    feats = rng.normal(size=(g.n_points, c_in)).astype(np.float32)
+    net = learned.CNN(H, c_in, 32, 2, shape["c_out"], seed=2)
+    params = net.get_params().astype(np.float64)
+    out = cnn_oracle.forward(params, feats, g.dims, **shape)
+    Yref = cnn_oracle.map_out(out, g.point, g.slot, 20, ne)
+    Y = net.start_block(feats, g, ne)
+    assert np.abs(Y - Yref).max() <= 1e-5 * np.abs(Yref).max()
+    V, cl, oc = _targets(H.n, rng)
+    L, grad = net.train_step(feats, g, V, cl, oc, lr=0.0, want_grad=True)
+    Lref, G = cnn_oracle.subspace_loss(Yref, V, cl, oc)
+    assert abs(L - Lref) <= 1e-6 * abs(Lref)
+    dout = cnn_oracle.map_grad(G, g.point, g.slot, 20, ne, g.n_points, shape["c_out"])
+    gref = cnn_oracle.backward(params, feats, g.dims, dout=dout, **shape)
+    assert np.abs(grad - gref).max() <= 1e-3 * np.abs(gref).max()
+    # lr = 0 leaves the network unchanged; a real step lowers the loss
+    assert np.array_equal(net.get_params(), params.astype(np.float32))
+    losses = [net.train_step(feats, g, V, cl, oc, lr=3e-3) for _ in range(30)]
+    assert losses[-1] < losses[0], losses
