@@ -187,14 +187,17 @@ typedef struct nh_blanczos_opts {
   int32_t conv_check_skip;      /* block steps between checks */
   uint64_t seed;                /* random start block seed */
   const nh_cplx *start_block;   /* optional host block [block_size][N] */
+  int32_t stop_rule;            /* as nh_lanczos_opts.stop_rule; counts
+                                   distinct levels (degenerate copies once) */
   int32_t want_vectors;
 } nh_blanczos_opts;
 
 /* Block Lanczos without reorthogonalisation (eigensolve_blk_lanczos,
  * lanczos_blk.c:2145-...): block three-term recurrence with a QR of each
  * new block, Ritz values of the block tridiagonal in the windows, Ritz
- * vectors by a second pass, FP64 residuals.  Degenerate levels (Kramers
- * pairs) come out as separate pairs. */
+ * vectors by a second pass, FP64 residuals.  Degenerate copies of a level
+ * (Kramers pairs) count once and give one eigenvector, as in the
+ * single-vector solver. */
 nh_status nh_eigensolve_block_lanczos(nh_context *ctx,
                                       const nh_blanczos_opts *opts,
                                       nh_eig_result *res);
@@ -272,6 +275,9 @@ nh_status nh_cnn_train_step(nh_cnn *cnn, const float *feat, int gx, int gy,
                             int n_targets, const int32_t *cluster,
                             int n_states, const int32_t *out_cluster,
                             double lr, double *loss, float *grad);
+/* Rebind the network to another context (another structure) on the same
+ * device: training runs over many structures with one network. */
+nh_status nh_cnn_bind(nh_cnn *cnn, nh_context *ctx);
 /* Start block [n_states][N] predicted for the context's structure. */
 nh_status nh_cnn_start_block(nh_cnn *cnn, const float *feat, int gx, int gy,
                              int gz, const int32_t *atom_point,

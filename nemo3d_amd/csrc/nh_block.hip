@@ -53,6 +53,7 @@ class BlockLanczos {
   nh_status orthonormalise(double2 *W, double2 *Qout, std::vector<cd> &R);
   nh_status step(int j, bool first_pass);
   void analyse(const nh_blanczos_opts *o, double hnorm);
+  bool window_done(int w, int req, int rule) const;
 
   nh_context *c_;
   std::vector<void *> mem_;
@@ -199,10 +200,13 @@ void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
     if (req[w] <= 0) continue;
     const double lo = win[w][0], hi = win[w][1];
     const int klo = band_count(T_, lo, work), khi = band_count(T_, hi, work);
-    const int want = req[w] + 8;
+    // candidates nearest the gap; every level may come as a degenerate pair
+    const int want = 2 * req[w] + 8;
     int k0 = klo, k1 = khi;
     if (w == 0) k0 = std::max(klo, khi - want);
     else k1 = std::min(khi, klo + want);
+    double prev = 0.0;
+    bool have_prev = false;
     for (int k = k0; k < k1; ++k) {
       double a = lo, b = hi;
       for (int it = 0; it < 200 && b - a > 1e-13 * (1.0 + std::fabs(a)); ++it) {
@@ -211,25 +215,46 @@ void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
         else a = mid;
       }
       const double ev = 0.5 * (a + b);
+      // copies of one level (Kramers partners, converged duplicates) count once
+      if (have_prev && std::fabs(ev - prev) < resolution) continue;
+      prev = ev;
+      have_prev = true;
       band_eigvec(T_, ev, x);
       // residual estimate ||B_m s_m|| (s_m: last block of the Ritz vector)
       double r2 = 0.0;
       for (int rr = 0; rr < p; ++rr) {
-        cd s = 0.0;
-        for (int cc = 0; cc < p; ++cc) s += Bm[rr * p + cc] * x[n - p + cc];
-        r2 += std::norm(s);
+        cd sm = 0.0;
+        for (int cc = 0; cc < p; ++cc) sm += Bm[rr * p + cc] * x[n - p + cc];
+        r2 += std::norm(sm);
       }
       const bool ok = std::sqrt(r2) <= tol;
       lastRitz_[w].push_back(ev);
       lastConv_[w].push_back(ok);
       if (ok) {
         bool known = false;
-        for (const auto &cv : conv_)
-          if (std::fabs(cv.value - ev) < resolution && cv.window == w) known = true;
+        for (auto &cv : conv_)
+          if (std::fabs(cv.value - ev) < resolution && cv.window == w) {
+            known = true;
+            cv.value = ev;  // keep the latest (most accurate) Ritz value
+          }
         if (!known) conv_.push_back({ev, T_.m, w});
       }
     }
   }
+}
+
+bool BlockLanczos::window_done(int w, int req, int rule) const {
+  if (req <= 0) return true;
+  if (rule == 0) {
+    int cnt = 0;
+    for (const auto &cv : conv_) cnt += cv.window == w;
+    return cnt >= req;
+  }
+  const auto &r = lastRitz_[w];
+  if ((int)r.size() < req) return false;
+  for (int t = 0; t < req; ++t)
+    if (!lastConv_[w][w == 0 ? r.size() - 1 - t : t]) return false;
+  return true;
 }
 
 nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
@@ -294,15 +319,8 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
       }
       hnorm = g;
       analyse(o, hnorm);
-      auto wdone = [&](int w, int req) {
-        if (req <= 0) return true;
-        const auto &r = lastRitz_[w];
-        if ((int)r.size() < req) return false;
-        for (int t = 0; t < req; ++t)
-          if (!lastConv_[w][w == 0 ? r.size() - 1 - t : t]) return false;
-        return true;
-      };
-      done = wdone(0, o->num_eig_vb) && wdone(1, o->num_eig_cb);
+      done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+             window_done(1, o->num_eig_cb, o->stop_rule);
     }
   }
   if (!done) analyse(o, hnorm);
@@ -310,6 +328,16 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
   std::vector<double> pick;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};
   for (int w = 0; w < 2; ++w) {
+    if (o->stop_rule == 0) {
+      // the converged levels of the window nearest the gap
+      std::vector<double> in;
+      for (const auto &cv : conv_)
+        if (cv.window == w) in.push_back(cv.value);
+      std::sort(in.begin(), in.end());
+      if (w == 0) std::reverse(in.begin(), in.end());
+      for (int t = 0; t < (int)in.size() && t < req[w]; ++t) pick.push_back(in[t]);
+      continue;
+    }
     const auto &r = lastRitz_[w];
     for (int t = 0; t < (int)r.size() && t < req[w]; ++t) {
       const int idx = w == 0 ? (int)r.size() - 1 - t : t;

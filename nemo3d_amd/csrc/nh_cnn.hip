@@ -24,6 +24,7 @@
 
 struct nh_cnn {
   nh_context *ctx = nullptr;
+  int device = 0;  // the context may be destroyed before the network
   int c_in = 0, width = 0, layers = 0, c_out = 0;
   int64_t n_params = 0;
   std::vector<int64_t> w_off, b_off;  // per layer (conv layers, then head)
@@ -433,6 +434,7 @@ nh_status nh_cnn_create(nh_context *ctx, int c_in, int width, int layers,
   NH_HIP(hipSetDevice(ctx->device));
   nh_cnn *c = new nh_cnn();
   c->ctx = ctx;
+  c->device = ctx->device;
   c->c_in = c_in;
   c->width = width;
   c->layers = layers;
@@ -486,7 +488,7 @@ nh_status nh_cnn_create(nh_context *ctx, int c_in, int width, int layers,
 
 void nh_cnn_destroy(nh_cnn *c) {
   if (!c) return;
-  hipSetDevice(c->ctx->device);
+  hipSetDevice(c->device);
   for (float *q : c->pre) hipFree(q);
   for (float *q : c->act) hipFree(q);
   void *ps[] = {c->master, c->bf, c->grad, c->m1, c->m2, c->in, c->out,
@@ -540,6 +542,16 @@ nh_status nh_cnn_backward(nh_cnn *c, const float *feat, int gx, int gy, int gz,
   NH_HIP(hipMemcpyAsync(grad, c->grad, c->n_params * sizeof(float),
                         hipMemcpyDeviceToHost, c->ctx->stream));
   NH_HIP(hipStreamSynchronize(c->ctx->stream));
+  return NH_OK;
+}
+
+nh_status nh_cnn_bind(nh_cnn *c, nh_context *ctx) {
+  NH_CHECK(c && ctx, NH_ERR_ARG, "nh_cnn_bind: null argument");
+  NH_CHECK(ctx->device == c->device, NH_ERR_ARG,
+           "nh_cnn_bind: context on another device");
+  // every network call synchronises its stream before returning, so no
+  // work is pending on the previous context's stream
+  c->ctx = ctx;
   return NH_OK;
 }
 

@@ -73,8 +73,8 @@ def local_strain(structure):
 
 def features(tbmodel, gmap, dot_mask=None):
     """(n_points, C_IN) float32 features.  dot_mask: per-atom bool (atoms
-    of the dot/wetting layer); default: cations of the model's second-
-    and-later cation species (the dot material)."""
+    of the dot/wetting layer); default: the structure's dot region, or
+    else every cation that is not Ga."""
     s = tbmodel.structure
     elements = s.elements
     cat_el = sorted({elements[k] for k in np.unique(s.species)
@@ -89,8 +89,10 @@ def features(tbmodel, gmap, dot_mask=None):
     e = local_strain(s)
     f[gmap.point[cat], len(cat_el) + 1:len(cat_el) + 7] = (e[cat] * 10.0).astype(np.float32)
     if dot_mask is None:
-        host = elements[s.species[np.nonzero(cat)[0][0]]] if False else None
-        dot_mask = cat & (np.array(elements)[s.species] != "Ga")
+        if s.dot_region is not None:
+            dot_mask = cat & s.dot_region
+        else:
+            dot_mask = cat & (np.array(elements)[s.species] != "Ga")
     f[gmap.point[dot_mask], len(cat_el) + 7] = 1.0
     pos = s.positions()
     if dot_mask.any():
@@ -128,6 +130,11 @@ class CNN:
                                             c_out, seed, ctypes.byref(self.h)))
         self.c_in, self.width, self.layers, self.c_out = c_in, width, layers, c_out
         self.n_params = int(self.lib.nh_cnn_num_params(self.h))
+
+    def bind(self, owner):
+        """Use the network on another structure (context) of the device."""
+        native.check(self.lib.nh_cnn_bind(self.h, owner.ctx))
+        self.owner = owner
 
     def close(self):
         if self.h:

@@ -81,6 +81,7 @@ class NHBLanczosOpts(ctypes.Structure):
         ("conv_check_skip", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("start_block", ctypes.c_void_p),
+        ("stop_rule", ctypes.c_int32),
         ("want_vectors", ctypes.c_int32),
     ]
 
@@ -156,6 +157,7 @@ SIGNATURES = {
                                          ctypes.c_int, P, P, P, ctypes.c_int,
                                          P, ctypes.c_int, P, ctypes.c_double,
                                          P, P]),
+    "nh_cnn_bind": (ctypes.c_int, [P, P]),
     "nh_cnn_start_block": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, P, P, ctypes.c_int, P,
                                           ctypes.c_int]),

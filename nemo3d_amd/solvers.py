@@ -79,9 +79,11 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
                              erange_vb=(-0.6, 0.2), erange_cb=(1.4, 2.0),
                              max_iter=2000, tol=1e-8, start_iter=20, skip=10,
                              seed=1, start_block=None, want_vectors=True,
-                             capacity=64, return_vectors=False):
+                             capacity=64, return_vectors=False,
+                             stop_rule="window"):
     """Block Lanczos (eigensolve_blk_lanczos, NEMO_3D/eigen/src/
-    lanczos_blk.c:2145-...) with BL_size = block_size."""
+    lanczos_blk.c:2145-...) with BL_size = block_size.  num_vb / num_cb
+    count distinct levels; stop_rule as in eigensolve_lanczos."""
     lib = native.load()
     o = native.NHBLanczosOpts()
     o.block_size = int(block_size)
@@ -98,6 +100,7 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
         sb = np.ascontiguousarray(start_block, dtype=np.complex128)
         assert sb.shape == (block_size, H.n)
         o.start_block = native.ptr(sb)
+    o.stop_rule = {"window": 0, "nearest": 1}[stop_rule]
     o.want_vectors = int(bool(want_vectors))
     ev = np.zeros(capacity)
     rs = np.full(capacity, -1.0)

@@ -68,6 +68,7 @@ class Structure:
         self.lattice = np.asarray(lattice, dtype=np.float64)
         self.periodic = np.asarray(periodic, dtype=bool)
         self.wrapped = wrapped
+        self.dot_region = None  # per atom: inside the dot / wetting layer
 
     @property
     def n_atoms(self):
@@ -193,8 +194,9 @@ def dot_structure(box_nm, lattice, shape="dome", base_nm=20.0, height_nm=5.0,
     wl_top = z0 + wl_ml * 0.5 * lattice[2]
     rng = np.random.default_rng(seed)
 
-    def occ(pos, m, cell):
-        # pos is measured from the cell-range origin (cells start at 1)
+    def in_region(pos):
+        """Dot or wetting layer; pos measured from the cell-range origin
+        (cells start at 1)."""
         p = pos - lattice * 1.0
         x, y, z = p[:, 0] - cx, p[:, 1] - cy, p[:, 2]
         if shape == "dome":
@@ -211,10 +213,12 @@ def dot_structure(box_nm, lattice, shape="dome", base_nm=20.0, height_nm=5.0,
         else:
             raise ValueError(shape)
         wl = (z >= z0) & (z < wl_top)
-        region = inside | wl
+        return inside | wl
+
+    def occ(pos, m, cell):
         cat = m < 4
         out = np.where(cat, 0, 1)
-        in_dot = region & cat
+        in_dot = in_region(pos) & cat
         if alloy_cation is not None and alloy_x < 1.0:
             draw = rng.random(pos.shape[0]) < alloy_x
             out = np.where(in_dot & draw, 2, out)
@@ -223,4 +227,6 @@ def dot_structure(box_nm, lattice, shape="dome", base_nm=20.0, height_nm=5.0,
             out = np.where(in_dot, 2, out)
         return out
 
-    return build((1, 1, 1), 1 + ncell, occ, elements, lattice)
+    st = build((1, 1, 1), 1 + ncell, occ, elements, lattice)
+    st.dot_region = in_region(st.positions(strained=False))
+    return st
