@@ -50,7 +50,12 @@ class BlockLanczos {
   nh_status gram(const double2 *X, const double2 *Y, std::vector<cd> &G);
   nh_status gemm(const double2 *X, int qx, double2 *Y, int qy,
                  const std::vector<cd> &M, double beta);
-  nh_status orthonormalise(double2 *W, double2 *Qout, std::vector<cd> &R);
+  struct Factors {
+    std::vector<cd> R1, R2;
+  };
+  nh_status orthonormalise(double2 *&W, double2 *&Qout, std::vector<cd> &R,
+                           Factors *f = nullptr);
+  nh_status replay_qr(double2 *&W, double2 *&Qout, const Factors &f);
   nh_status step(int j, bool first_pass);
   void analyse(const nh_blanczos_opts *o, double hnorm);
   bool window_done(int w, int req, int rule) const;
@@ -64,6 +69,7 @@ class BlockLanczos {
   int nparts_ = 0;
   BandT T_;
   std::vector<cd> Bnext_;
+  std::vector<Factors> qr_;  // Cholesky QR factors of every step
   std::vector<Pair> conv_;
   std::vector<double> lastRitz_[2];
   std::vector<char> lastConv_[2];
@@ -121,26 +127,43 @@ nh_status BlockLanczos::gemm(const double2 *X, int qx, double2 *Y, int qy,
   return NH_OK;
 }
 
-// Qout R = W (CholQR, twice); R upper triangular, returned.
-nh_status BlockLanczos::orthonormalise(double2 *W, double2 *Qout,
-                                       std::vector<cd> &R) {
+// Qout R = W by Cholesky QR twice: Q1 = W R1^-1, Q = Q1 R2^-1, R = R2 R1
+// (upper triangular).  The factors are kept so that the eigenvector pass
+// repeats exactly these two products (a single product with R^-1 differs in
+// rounding, and the difference grows along the recurrence).  The result ends
+// in Qout, W holds Q1 afterwards (the pointers are swapped, no product is
+// done in place).
+nh_status BlockLanczos::orthonormalise(double2 *&W, double2 *&Qout,
+                                       std::vector<cd> &R, Factors *f) {
   const int p = p_;
-  std::vector<cd> G;
-  R.assign(p * p, 0.0);
-  for (int k = 0; k < p; ++k) R[k * p + k] = 1.0;
-  double2 *src = W;
+  std::vector<cd> G[2];
   for (int pass = 0; pass < 2; ++pass) {
-    NH_TRY(gram(src, src, G));
+    double2 *src = pass == 0 ? W : Qout;
+    double2 *dst = pass == 0 ? Qout : W;
+    NH_TRY(gram(src, src, G[pass]));
     // tiny diagonal lift keeps Cholesky alive for a nearly dependent block
     double tr = 0.0;
-    for (int k = 0; k < p; ++k) tr += G[k * p + k].real();
-    for (int k = 0; k < p; ++k) G[k * p + k] += 1e-28 * tr;
-    NH_CHECK(cholesky_upper(p, G), NH_ERR_NOCONV,
+    for (int k = 0; k < p; ++k) tr += G[pass][k * p + k].real();
+    for (int k = 0; k < p; ++k) G[pass][k * p + k] += 1e-28 * tr;
+    NH_CHECK(cholesky_upper(p, G[pass]), NH_ERR_NOCONV,
              "block Lanczos: block became rank deficient");
-    NH_TRY(gemm(src, p, Qout, p, inv_upper(p, G), 0.0));
-    R = matmul(p, G, R);
-    src = Qout;
+    NH_TRY(gemm(src, p, dst, p, inv_upper(p, G[pass]), 0.0));
   }
+  std::swap(W, Qout);  // result (written to W by the second pass) -> Qout
+  R = matmul(p, G[1], G[0]);
+  if (f) {
+    f->R1 = G[0];
+    f->R2 = G[1];
+  }
+  return NH_OK;
+}
+
+// Replays orthonormalise with stored factors: Qout = (W R1^-1) R2^-1.
+nh_status BlockLanczos::replay_qr(double2 *&W, double2 *&Qout,
+                                  const Factors &f) {
+  NH_TRY(gemm(W, p_, Qout, p_, inv_upper(p_, f.R1), 0.0));
+  NH_TRY(gemm(Qout, p_, W, p_, inv_upper(p_, f.R2), 0.0));
+  std::swap(W, Qout);
   return NH_OK;
 }
 
@@ -175,10 +198,12 @@ nh_status BlockLanczos::step(int j, bool first_pass) {
   std::vector<cd> R;
   // Qprev <- new block (the old Q_{j-1} is no longer needed)
   if (first_pass) {
-    NH_TRY(orthonormalise(W_, Qprev_, R));
+    Factors f;
+    NH_TRY(orthonormalise(W_, Qprev_, R, &f));
     T_.B.push_back(R);
+    qr_.push_back(f);
   } else {
-    NH_TRY(gemm(W_, p, Qprev_, p, inv_upper(p, T_.B[j]), 0.0));
+    NH_TRY(replay_qr(W_, Qprev_, qr_[j]));
   }
   std::swap(Qprev_, Q_);  // Q_ = Q_{j+1}, Qprev_ = Q_j
   return NH_OK;

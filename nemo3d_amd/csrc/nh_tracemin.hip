@@ -190,8 +190,11 @@ nh_status Tracemin::run(const nh_tracemin_opts *o, nh_eig_result *res) {
         v.im = (double)(st >> 11) / 9007199254740992.0;
       }
     }
-    NH_HIP(hipMemcpy(Y, S.data(), (size_t)s_ * n_ * sizeof(double2),
-                     hipMemcpyHostToDevice));
+    NH_CHECK(Y && !S.empty(), NH_ERR_STATE,
+             "TRACEMIN: start block of %d x %lld not allocated", s_, (long long)n_);
+    NH_HIP(hipMemcpyAsync(Y, S.data(), (size_t)s_ * n_ * sizeof(double2),
+                          hipMemcpyHostToDevice, c->stream));
+    NH_HIP(hipStreamSynchronize(c->stream));
   }
   const int ne = o->num_eig;
   std::vector<double> theta, lam(s_), resid(s_);

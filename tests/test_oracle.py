@@ -1,5 +1,6 @@
 """The CPU oracle against the reference's golden data and closed forms."""
 import os
+import sys
 
 import numpy as np
 import pytest
@@ -153,3 +154,57 @@ def test_closed_box_hermitian_and_surface():
     H0 = oracle.assemble(M0.arrays())
     changed = np.abs(H.onsite - H0.onsite).max(axis=1) > 0
     assert np.array_equal(changed, s.surface_mask())
+
+
+def _ref_sk_check(data):
+    """Oracle bond (or_sk_bond) against the reference routine's outputs:
+    the anion-row block is T^T in the reference's storage, the cation-row
+    block T; the diagonal corrections go to the row atom."""
+    worst_t = worst_c = 0.0
+    for k in range(len(data["nb"])):
+        T, ca, cc = oracle.sk_bond(data["par"][k], data["strain"][k], data["a0"][k],
+                                   data["nnv"][k], data["nnv0"][k])
+        want = T.T if data["anion_row"][k] else T
+        worst_t = max(worst_t, np.abs(data["ho"][k] - want).max())
+        corr = ca if data["anion_row"][k] else cc
+        worst_c = max(worst_c, np.abs(data["hd"][k] - corr).max())
+    return worst_t, worst_c
+
+
+def test_strained_bonds_match_reference_fixture():
+    """Strained SK blocks + Boykin corrections of random bonds, as written by
+    the reference's own h3d_offdiag_diag_corr (tests/golden/
+    make_ref_sk_fixture.py), both band models and row roles."""
+    d = np.load(os.path.join(GOLDEN, "ref_sk_bonds.npz"))
+    assert np.abs(d["hd"]).max() > 1e-2  # the strain corrections are exercised
+    worst_t, worst_c = _ref_sk_check(d)
+    assert worst_t < 1e-13 and worst_c < 1e-14, (worst_t, worst_c)
+
+
+def test_strained_bonds_detect_planted_error():
+    """A mistake planted in the oracle's input (one strain exponent) breaks
+    the agreement with the reference outputs."""
+    d = dict(np.load(os.path.join(GOLDEN, "ref_sk_bonds.npz")))
+    d["strain"] = d["strain"].copy()
+    d["strain"][:, 4] += 0.1  # eta of pp sigma
+    worst_t, worst_c = _ref_sk_check(d)
+    assert worst_t > 1e-4
+
+
+def test_strained_bonds_match_reference_harness(tmp_path):
+    """Fresh random bonds through the compiled reference routine
+    (oracle/_ref/ref_sk, built by oracle/ref_build.sh where the reference
+    tree is present)."""
+    harness = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "ref_sk")
+    if not os.path.exists(harness):
+        pytest.skip("reference harness not built (no reference tree)")
+    sys.path.insert(0, GOLDEN)
+    import make_ref_sk_fixture as mk
+    cs = mk.cases(seed=11, n=16)
+    ho, hd = mk.run(cs, harness)
+    d = dict(nb=np.array([c[0] for c in cs]), anion_row=np.array([c[1] for c in cs]),
+             a0=np.array([c[2] for c in cs]), nnv=np.array([c[3] for c in cs]),
+             nnv0=np.array([c[4] for c in cs]), par=np.array([c[5] for c in cs]),
+             strain=np.array([c[6] for c in cs]), ho=ho, hd=hd)
+    worst_t, worst_c = _ref_sk_check(d)
+    assert worst_t < 1e-13 and worst_c < 1e-14, (worst_t, worst_c)
