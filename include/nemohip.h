@@ -77,6 +77,13 @@ nh_status nh_context_create(const nh_model *model, int device,
 void nh_context_destroy(nh_context *ctx);
 int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
 
+/* Kernel variants (chosen by measured throughput; both are exact to the
+ * same rounding).  NH_SET_HV_STAGED: 1 (default) stages the neighbours'
+ * vector entries of the stored-block H·v in LDS, 0 reads them from the
+ * caches by every lane. */
+enum { NH_SET_HV_STAGED = 1 };
+nh_status nh_context_set(nh_context *ctx, int what, int value);
+
 /* Assemble H on the device in the given layout and precision.  Replaces
  * the first-call assembly of Hmult_spds_col (h_cvectr_mult.c:1134-1886)
  * with its storage into HamZincBlende (Hmult_spds_col_prep_store,

@@ -151,6 +151,17 @@ using namespace nh;
 
 extern "C" {
 
+nh_status nh_context_set(nh_context *ctx, int what, int value) {
+  NH_CHECK(ctx, NH_ERR_ARG, "null context");
+  switch (what) {
+    case NH_SET_HV_STAGED:
+      ctx->hv_staged = value != 0;
+      return NH_OK;
+    default:
+      NH_CHECK(false, NH_ERR_ARG, "nh_context_set: unknown setting %d", what);
+  }
+}
+
 const char *nh_last_error(void) { return g_err; }
 
 nh_status nh_context_create(const nh_model *model, int device,

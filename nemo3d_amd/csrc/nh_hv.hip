@@ -42,21 +42,31 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
   }
 }
 
-template <int NSP, typename TB, typename TA>
+template <int NSP, typename TB, typename TA, bool STAGED>
 __global__ __launch_bounds__(kThreads) void k_hv_blocks(
     int A, const double *__restrict__ onsite, const TB *__restrict__ blocks,
     const int32_t *__restrict__ nbr, double shift,
     const double2 *__restrict__ x, double2 *__restrict__ y, int64_t ld) {
   constexpr int NB = NSP * 10;
-  const int atom = blockIdx.x * kAtomsPerBlock + threadIdx.x / 10;
+  const int la = threadIdx.x / 10;
+  const int atom = blockIdx.x * kAtomsPerBlock + la;
   const int o = threadIdx.x % 10;
-  if (atom >= A) return;
   const double2 *xc = x + (int64_t)blockIdx.y * ld;
   double2 *yc = y + (int64_t)blockIdx.y * ld;
   double acc_re[NSP], acc_im[NSP];
   double2 xi[NSP];
-  hv_row<NSP, TB, TA>(atom, o, onsite, blocks, nbr, shift, xc, acc_re, acc_im,
-                      xi);
+  if constexpr (STAGED) {
+    __shared__ double2 sx[kAtomsPerBlock * 4 * NB];
+    if (atom < A) stage_nbr_x<NSP>(atom, o, nbr, xc, sx + la * 4 * NB);
+    __syncthreads();
+    if (atom >= A) return;
+    hv_row_staged<NSP, TB, TA>(atom, o, onsite, blocks, shift, xc,
+                               sx + la * 4 * NB, acc_re, acc_im, xi);
+  } else {
+    if (atom >= A) return;
+    hv_row<NSP, TB, TA>(atom, o, onsite, blocks, nbr, shift, xc, acc_re,
+                        acc_im, xi);
+  }
   const int64_t base = (int64_t)atom * NB;
 #pragma unroll
   for (int s = 0; s < NSP; ++s)
@@ -96,25 +106,26 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
   const dim3 grid((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock, ncol);
   const double2 *xd = reinterpret_cast<const double2 *>(x);
   double2 *yd = reinterpret_cast<double2 *>(y);
+  const bool st = c->hv_staged;
+#define HVB(NSP, TB, TA, BLK)                                                  \
+  do {                                                                       \
+    if (st)                                                                  \
+      hipLaunchKernelGGL((k_hv_blocks<NSP, TB, TA, true>), grid,             \
+                         dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite, \
+                         BLK, c->nbr, shift, xd, yd, ld);                    \
+    else                                                                     \
+      hipLaunchKernelGGL((k_hv_blocks<NSP, TB, TA, false>), grid,            \
+                         dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite, \
+                         BLK, c->nbr, shift, xd, yd, ld);                    \
+  } while (0)
   if (c->precision == NH_PREC_FP64) {
-    if (c->nb == 20)
-      hipLaunchKernelGGL((k_hv_blocks<2, double, double>), grid, dim3(kThreads), 0,
-                         c->stream, c->n_atoms, c->onsite, c->blocks64, c->nbr,
-                         shift, xd, yd, ld);
-    else
-      hipLaunchKernelGGL((k_hv_blocks<1, double, double>), grid, dim3(kThreads), 0,
-                         c->stream, c->n_atoms, c->onsite, c->blocks64, c->nbr,
-                         shift, xd, yd, ld);
+    if (c->nb == 20) HVB(2, double, double, c->blocks64);
+    else HVB(1, double, double, c->blocks64);
   } else {
-    if (c->nb == 20)
-      hipLaunchKernelGGL((k_hv_blocks<2, float, float>), grid, dim3(kThreads), 0,
-                         c->stream, c->n_atoms, c->onsite, c->blocks32, c->nbr,
-                         shift, xd, yd, ld);
-    else
-      hipLaunchKernelGGL((k_hv_blocks<1, float, float>), grid, dim3(kThreads), 0,
-                         c->stream, c->n_atoms, c->onsite, c->blocks32, c->nbr,
-                         shift, xd, yd, ld);
+    if (c->nb == 20) HVB(2, float, float, c->blocks32);
+    else HVB(1, float, float, c->blocks32);
   }
+#undef HVB
   NH_HIP(hipGetLastError());
   return NH_OK;
 }

@@ -8,8 +8,12 @@
 // row and applied to both spin components (Hv_stored_20_Hzb,
 // Hmult_stored.c:176-607, which applies Hu to rows i and i+10).  Each lane
 // reads its block row as 10 contiguous values; the 10 lanes of an atom read
-// one contiguous 800 B (FP64) / 400 B (FP32) block.  Neighbour vector
-// entries are read through the L1/L2 caches by all lanes of the atom.
+// one contiguous 800 B (FP64) / 400 B (FP32) block.
+//
+// Neighbour vector entries: either read by every lane straight from
+// global memory (through L1/L2; hv_row), or first staged in LDS by the 10
+// lanes of the atom, each fetching 8 of the 80 values once (stage_nbr_x +
+// hv_row_staged).
 //
 // FP32 storage: each block product is formed in FP32 (values and the FP32
 // image of x) and added to an FP64 row accumulator.
@@ -34,16 +38,15 @@ __device__ __forceinline__ void sp_partners(int o, int p[3], int k[3]) {
   }
 }
 
-// acc[s] = ((H - shift I) x) at row (atom, spin s, orbital o).
-template <int NSP, typename TB, typename TA>
-__device__ __forceinline__ void hv_row(int atom, int o,
-                                       const double *__restrict__ onsite,
-                                       const TB *__restrict__ blocks,
-                                       const int32_t *__restrict__ nbr,
-                                       double shift,
-                                       const double2 *__restrict__ xc,
-                                       double acc_re[NSP], double acc_im[NSP],
-                                       double2 xi[NSP]) {
+// Onsite part of row (atom, spin s, orbital o): (diagonal - shift) x,
+// dangling-bond s-p couplings and spin-orbit.  Initialises acc.
+template <int NSP>
+__device__ __forceinline__ void hv_onsite(int atom, int o,
+                                          const double *__restrict__ onsite,
+                                          double shift,
+                                          const double2 *__restrict__ xc,
+                                          double acc_re[NSP], double acc_im[NSP],
+                                          double2 xi[NSP]) {
   constexpr int NB = NSP * 10;
   const int64_t base = (int64_t)atom * NB;
   const double *os = onsite + (size_t)atom * kOnsiteStride;
@@ -93,32 +96,89 @@ __device__ __forceinline__ void hv_row(int atom, int o,
       acc_im[NSP - 1] += lam * xa[2].y + lam * xa[3].x;
     }
   }
-  // neighbour blocks
+}
+
+// One neighbour block row applied to both spins: acc += B[o][:] x_j.
+template <int NSP, typename TB, typename TA>
+__device__ __forceinline__ void hv_block(const TB *__restrict__ B,
+                                         const double2 *xj, double acc_re[NSP],
+                                         double acc_im[NSP]) {
+  TA br[NSP], bi[NSP];
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) br[s] = bi[s] = TA(0);
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    const TA b = TA(B[k]);
+#pragma unroll
+    for (int s = 0; s < NSP; ++s) {
+      const double2 v = xj[s * 10 + k];
+      br[s] += b * TA(v.x);
+      bi[s] += b * TA(v.y);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NSP; ++s) {
+    acc_re[s] += double(br[s]);
+    acc_im[s] += double(bi[s]);
+  }
+}
+
+// acc[s] = ((H - shift I) x) at row (atom, spin s, orbital o).
+template <int NSP, typename TB, typename TA>
+__device__ __forceinline__ void hv_row(int atom, int o,
+                                       const double *__restrict__ onsite,
+                                       const TB *__restrict__ blocks,
+                                       const int32_t *__restrict__ nbr,
+                                       double shift,
+                                       const double2 *__restrict__ xc,
+                                       double acc_re[NSP], double acc_im[NSP],
+                                       double2 xi[NSP]) {
+  constexpr int NB = NSP * 10;
+  hv_onsite<NSP>(atom, o, onsite, shift, xc, acc_re, acc_im, xi);
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
     const int j = nbr[atom * 4 + n];
     if (j < 0) continue;
-    const TB *B = blocks + ((size_t)atom * 4 + n) * 100 + o * 10;
-    const double2 *xj = xc + (int64_t)j * NB;
-    TA br[NSP], bi[NSP];
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) br[s] = bi[s] = TA(0);
-#pragma unroll
-    for (int k = 0; k < 10; ++k) {
-      const TA b = TA(B[k]);
-#pragma unroll
-      for (int s = 0; s < NSP; ++s) {
-        const double2 v = xj[s * 10 + k];
-        br[s] += b * TA(v.x);
-        bi[s] += b * TA(v.y);
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < NSP; ++s) {
-      acc_re[s] += double(br[s]);
-      acc_im[s] += double(bi[s]);
-    }
+    hv_block<NSP, TB, TA>(blocks + ((size_t)atom * 4 + n) * 100 + o * 10,
+                          xc + (int64_t)j * NB, acc_re, acc_im);
   }
+}
+
+// Stages the x entries of the atom's four neighbours (4 x NSP*10 complex
+// values) in shared memory: lane o fetches entries o, o+10, ...; missing
+// neighbours are staged as zeros.  sx: the atom's 40*NSP slots.
+template <int NSP>
+__device__ __forceinline__ void stage_nbr_x(int atom, int o,
+                                            const int32_t *__restrict__ nbr,
+                                            const double2 *__restrict__ xc,
+                                            double2 *sx) {
+  constexpr int NB = NSP * 10;
+#pragma unroll
+  for (int t = 0; t < 4 * NSP; ++t) {
+    const int e = o + 10 * t;  // 0 .. 40*NSP-1
+    const int n = e / NB, k = e % NB;
+    const int j = nbr[atom * 4 + n];
+    sx[e] = j >= 0 ? xc[(int64_t)j * NB + k] : make_double2(0.0, 0.0);
+  }
+}
+
+// hv_row with the neighbour entries read from the staged copy.
+template <int NSP, typename TB, typename TA>
+__device__ __forceinline__ void hv_row_staged(int atom, int o,
+                                              const double *__restrict__ onsite,
+                                              const TB *__restrict__ blocks,
+                                              double shift,
+                                              const double2 *__restrict__ xc,
+                                              const double2 *sx,
+                                              double acc_re[NSP],
+                                              double acc_im[NSP],
+                                              double2 xi[NSP]) {
+  constexpr int NB = NSP * 10;
+  hv_onsite<NSP>(atom, o, onsite, shift, xc, acc_re, acc_im, xi);
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+    hv_block<NSP, TB, TA>(blocks + ((size_t)atom * 4 + n) * 100 + o * 10,
+                          sx + n * NB, acc_re, acc_im);
 }
 
 }  // namespace nh

@@ -72,6 +72,9 @@ struct nh_context {
   int32_t *bond_of = nullptr;   // [A][4] cation-owned bond id (ci*4+slot)
   int32_t *rev_slot = nullptr;  // [A][4] slot of this atom in the nbr's list
 
+  // kernel variants (nh_context_set)
+  bool hv_staged = true;  // stored-block H·v: neighbour x staged in LDS
+
   // assembled Hamiltonian
   int layout = -1;
   int precision = -1;

@@ -44,22 +44,33 @@ __device__ __forceinline__ double wg_sum320(double v) {
   return t;
 }
 
-template <int NSP, typename TB, typename TA>
+template <int NSP, typename TB, typename TA, bool STAGED>
 __global__ __launch_bounds__(kThreads) void k_lz_hv(
     int A, const double *__restrict__ onsite, const TB *__restrict__ blocks,
     const int32_t *__restrict__ nbr, const double2 *__restrict__ R,
     double2 *__restrict__ Q, double2 *__restrict__ Y,
     const double *__restrict__ beta_prev, double *__restrict__ part) {
   constexpr int NB = NSP * 10;
-  const int atom = blockIdx.x * kAtomsPerBlock + threadIdx.x / 10;
+  const int la = threadIdx.x / 10;
+  const int atom = blockIdx.x * kAtomsPerBlock + la;
   const int o = threadIdx.x % 10;
   const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
   double local = 0.0;
+  double acc_re[NSP], acc_im[NSP];
+  double2 xi[NSP];
+  if constexpr (STAGED) {
+    __shared__ double2 sx[kAtomsPerBlock * 4 * NB];
+    if (atom < A) stage_nbr_x<NSP>(atom, o, nbr, R, sx + la * 4 * NB);
+    __syncthreads();
+    if (atom < A)
+      hv_row_staged<NSP, TB, TA>(atom, o, onsite, blocks, 0.0, R,
+                                 sx + la * 4 * NB, acc_re, acc_im, xi);
+  } else {
+    if (atom < A)
+      hv_row<NSP, TB, TA>(atom, o, onsite, blocks, nbr, 0.0, R, acc_re, acc_im,
+                          xi);
+  }
   if (atom < A) {
-    double acc_re[NSP], acc_im[NSP];
-    double2 xi[NSP];
-    hv_row<NSP, TB, TA>(atom, o, onsite, blocks, nbr, 0.0, R, acc_re, acc_im,
-                        xi);
     const int64_t base = (int64_t)atom * NB;
 #pragma unroll
     for (int s = 0; s < NSP; ++s) {
@@ -359,10 +370,17 @@ nh_status LanczosRun::iterate(int k, bool red) {
     NH_HIP(hipEventCreate(&evp.second));
     NH_HIP(hipEventRecord(evp.first, c->stream));
   }
-#define LZ_HV(NSP, TB, BLK)                                                   \
-  hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB>), dim3(gridA), dim3(kThreads), 0,  \
-                     c->stream, c->n_atoms, c->onsite, BLK, c->nbr, R_, Q, Y_, \
-                     bprev, part)
+#define LZ_HV(NSP, TB, BLK)                                                    \
+  do {                                                                         \
+    if (c->hv_staged)                                                          \
+      hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB, true>), dim3(gridA),            \
+                         dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
+                         BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
+    else                                                                       \
+      hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB, false>), dim3(gridA),           \
+                         dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
+                         BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
+  } while (0)
 #define LZ_SK(NSP, R, SKP)                                                      \
   hipLaunchKernelGGL((k_lz_hv_sk<NSP, R>), dim3(gridA), dim3(kSkThreads), 0,    \
                      c->stream, c->n_cations, c->cations, c->n_anions,          \

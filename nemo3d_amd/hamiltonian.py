@@ -69,6 +69,10 @@ class Hamiltonian:
         native.check(native.load().nh_ham_build(self._ctx, layout, precision))
         self.layout, self.precision = layout, precision
 
+    def set(self, what, value):
+        """Kernel variant switch (nh_context_set)."""
+        native.check(native.load().nh_context_set(self._ctx, int(what), int(value)))
+
     def hv(self, x, shift=0.0, out=None):
         """y = (H - shift I) x for one vector (numpy host or DeviceArray)."""
         return self.hV(x, shift, out)

@@ -15,6 +15,7 @@ NH_OK = 0
 LAYOUT_BLOCKS, LAYOUT_SK, LAYOUT_MATRIX_FREE = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
+SET_HV_STAGED = 1
 
 _STATUS = {0: "NH_OK", 1: "NH_ERR_ARG", 2: "NH_ERR_HIP", 3: "NH_ERR_STATE",
            4: "NH_ERR_NOMEM", 5: "NH_ERR_NOCONV"}
@@ -124,6 +125,7 @@ P = ctypes.c_void_p
 SIGNATURES = {
     "nh_context_create": (ctypes.c_int, [P, ctypes.c_int, P]),
     "nh_context_destroy": (None, [P]),
+    "nh_context_set": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "nh_context_dim": (ctypes.c_int64, [P]),
     "nh_ham_build": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "nh_hv": (ctypes.c_int, [P, ctypes.c_double, P, P, ctypes.c_int]),

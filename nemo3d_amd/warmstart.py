@@ -1,0 +1,161 @@
+"""Learned warm start: training set from the plain solver, training on the
+device, and the start block for a new structure.
+
+Training structures are seeded dome dots (base 12-24 nm, height 3-7 nm, In
+fraction 0.6-1.0 drawn per cation site of the dot and wetting layer) in a
+GaAs box, relaxed by the device Keating VFF.  Their targets are the levels
+nearest the gap from the plain FP64 Lanczos solver (the reference's
+eigensolve_lanczos), each with its Kramers partner; the network learns to
+span each level's two-dimensional subspace.
+
+    python -m nemo3d_amd.warmstart train --dots 16 --out FILE.npz
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import learned, materials, model, native, structure, vff
+from .hamiltonian import Hamiltonian
+from .solvers import eigensolve_lanczos
+
+A_GAAS = 0.56532  # nm, GaAs unstrnd_cubic_cell_length of the material file
+COMPOUNDS = ["GaAs", "InAs"]
+
+
+def dot_model(box_nm, base_nm, height_nm, x_in=1.0, seed=0, shape="dome",
+              top_nm=None, db_shift=30.0):
+    """Dot + 1-ML wetting layer of In_x Ga_(1-x) As in a GaAs box."""
+    s = structure.dot_structure(
+        box_nm, (A_GAAS,) * 3, shape=shape, base_nm=base_nm,
+        height_nm=height_nm, top_nm=top_nm, wl_ml=1, alloy_x=x_in,
+        alloy_cation="Ga" if x_in < 1.0 else None, seed=seed)
+    return model.TBModel(s, COMPOUNDS, materials.BANDS_20_SPIN, passivate=True,
+                         db_shift=db_shift)
+
+
+def training_params(k, seed0=7000):
+    rng = np.random.default_rng(seed0 + k)
+    return dict(base_nm=float(rng.uniform(12, 24)),
+                height_nm=float(rng.uniform(3, 7)),
+                x_in=float(rng.uniform(0.6, 1.0)), seed=seed0 + k)
+
+
+def plain_levels(H, n_vb=4, n_cb=4, erange_vb=(0.3, 0.75),
+                 erange_cb=(1.3, 1.7), tol=1e-8, max_iter=40000):
+    """The n_vb highest valence and n_cb lowest conduction levels of the
+    plain FP64 Lanczos solver with their eigenvectors (host)."""
+    r = eigensolve_lanczos(H, num_vb=n_vb, num_cb=n_cb, erange_vb=erange_vb,
+                           erange_cb=erange_cb, max_iter=max_iter, tol=tol,
+                           start_iter=200, skip=100, stop_rule="nearest",
+                           return_vectors=True, capacity=n_vb + n_cb)
+    return r
+
+
+def sample(M, energies, vectors, owner=None):
+    """Features, grid map and targets (levels + Kramers partners, uploaded
+    to the device) of one training structure."""
+    g = learned.GridMap(M.structure)
+    f = learned.features(M, g)
+    ne = len(energies)
+    T = np.empty((2 * ne, M.n), dtype=np.complex128)
+    for e in range(ne):
+        T[2 * e] = vectors[e]
+        T[2 * e + 1] = learned.kramers_partner(vectors[e], M.nb)
+    owner = owner if owner is not None else Hamiltonian(M, layout=None)
+    dT = native.DeviceArray.from_host(owner, T)
+    return dict(owner=owner, feats=f, gmap=g, targets=dT,
+                cluster=np.repeat(np.arange(ne), 2).astype(np.int32),
+                out_cluster=np.arange(ne, dtype=np.int32),
+                energies=np.asarray(energies))
+
+
+def new_network(owner, c_in, seed=0):
+    return learned.CNN(owner, c_in, width=32, layers=4,
+                       c_out=learned.N_STATES * learned.N_AMP * 2, seed=seed)
+
+
+def train(net, samples, epochs=60, lr=2e-3, log=print):
+    """Adam over the training structures; returns the loss history."""
+    hist = []
+    for ep in range(epochs):
+        tot = 0.0
+        for smp in samples:
+            net.bind(smp["owner"])
+            tot += net.train_step(smp["feats"], smp["gmap"], smp["targets"],
+                                  smp["cluster"], smp["out_cluster"], lr)
+        hist.append(tot / len(samples))
+        if log and (ep % 10 == 0 or ep == epochs - 1):
+            log("epoch %d loss %.4f" % (ep, hist[-1]))
+    return hist
+
+
+def save(net, path, meta):
+    np.savez(path, params=net.get_params(), c_in=net.c_in, width=net.width,
+             layers=net.layers, c_out=net.c_out, meta=json.dumps(meta))
+
+
+def load(owner, path):
+    d = np.load(path, allow_pickle=False)
+    net = learned.CNN(owner, int(d["c_in"]), int(d["width"]), int(d["layers"]),
+                      int(d["c_out"]))
+    net.set_params(d["params"])
+    return net, json.loads(str(d["meta"]))
+
+
+def start_block(net, H, n_states=learned.N_STATES):
+    """Predicted start block (host, n_states x N) for H's structure,
+    orthonormalised columns."""
+    net.bind(H)
+    g = learned.GridMap(H.model.structure)
+    f = learned.features(H.model, g)
+    return net.start_block(f, g, n_states)
+
+
+def subspace_overlap(block, vectors, nb=20):
+    """Per target level: weight of the level's Kramers pair in the
+    orthonormalised block (1 = contained)."""
+    Q = np.linalg.qr(np.asarray(block).T)[0]
+    out = []
+    for v in vectors:
+        P = np.stack([v, learned.kramers_partner(v, nb)])
+        P = np.linalg.qr(P.T)[0]
+        out.append(float(np.linalg.norm(np.conj(P.T) @ Q) ** 2 / 2))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("--dots", type=int, default=16)
+    ap.add_argument("--box", type=float, nargs=3, default=(24.0, 24.0, 16.0))
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+    samples, meta = [], []
+    for k in range(a.dots):
+        p = training_params(k)
+        t0 = time.time()
+        M = dot_model(tuple(a.box), p["base_nm"], p["height_nm"], p["x_in"], p["seed"])
+        info = vff.relax(M)
+        H = Hamiltonian(M)
+        r = plain_levels(H)
+        if not r["converged"] or len(r["eigenvalues"]) < 8:
+            print("dot %d: plain solver did not converge, skipped" % k, flush=True)
+            continue
+        samples.append(sample(M, r["eigenvalues"], r["vectors"], owner=H))
+        meta.append(dict(p, atoms=M.n_atoms, vff_iter=info["iterations"],
+                         levels=[float(e) for e in r["eigenvalues"]],
+                         lanczos_iter=r["iterations"]))
+        print("dot %d: %d atoms, levels %s, %.1fs" % (
+            k, M.n_atoms, np.round(r["eigenvalues"], 4), time.time() - t0), flush=True)
+    net = new_network(samples[0]["owner"], samples[0]["feats"].shape[1])
+    hist = train(net, samples, a.epochs, a.lr,
+                 log=lambda m: print(m, flush=True))
+    save(net, a.out, dict(dots=meta, loss=hist, box_nm=list(a.box)))
+
+
+if __name__ == "__main__":
+    main()
