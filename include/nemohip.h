@@ -90,6 +90,14 @@ nh_status nh_context_set(nh_context *ctx, int what, int value);
  * h_cvectr_mult.c:1888-2082). */
 nh_status nh_ham_build(nh_context *ctx, int layout, int precision);
 
+/* Switch the arithmetic of later products between FP64 and FP32 without
+ * re-assembling (the FP64 values are always kept; the FP32 copies are made
+ * on first use).  The mixed-precision solvers run their bulk iterations in
+ * FP32 and their final refinement in FP64 on one context.  No reference
+ * counterpart (the reference picks FP32 storage at compile time,
+ * HamMemorySave). */
+nh_status nh_ham_set_precision(nh_context *ctx, int precision);
+
 /* y = (H - shift I) x.  Replaces Hmult_spds_complete
  * (h_cvectr_mult.c:262-667) / Hmult_spds_col_stored (Hmult_stored.c:73-172). */
 nh_status nh_hv(nh_context *ctx, double shift, const nh_cplx *x, nh_cplx *y,

@@ -178,6 +178,12 @@ nh_status nh_ham_build(nh_context *ctx, int layout, int precision) {
   return assemble(ctx, layout, precision);
 }
 
+nh_status nh_ham_set_precision(nh_context *ctx, int precision) {
+  NH_TRY(check_ctx(ctx));
+  NH_HIP(hipSetDevice(ctx->device));
+  return select_precision(ctx, precision);
+}
+
 nh_status nh_hv(nh_context *ctx, double shift, const nh_cplx *x, nh_cplx *y,
                 int ptr_kind) {
   return nh_hV(ctx, shift, x, y, 1, ctx ? ctx->n : 0, ptr_kind);

@@ -289,4 +289,36 @@ nh_status assemble(nh_context *c, int layout, int precision) {
   return NH_OK;
 }
 
+nh_status select_precision(nh_context *c, int precision) {
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
+           NH_ERR_STATE, "build the Hamiltonian first");
+  NH_CHECK(precision == NH_PREC_FP64 || precision == NH_PREC_FP32, NH_ERR_ARG,
+           "precision %d not available", precision);
+  const int A = c->n_atoms;
+  if (precision == NH_PREC_FP32) {
+    // the FP64 values are always kept: derive the FP32 copies once
+    if (!c->onsite32) {
+      NH_HIP(hipMalloc(&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
+      hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
+                         (int64_t)A * kOnsiteStride, c->onsite, c->onsite32);
+    }
+    if (c->layout == NH_LAYOUT_BLOCKS && !c->blocks32) {
+      const size_t nblk = (size_t)A * 4 * 100;
+      NH_HIP(hipMalloc(&c->blocks32, nblk * sizeof(float)));
+      hipLaunchKernelGGL(k_convert<float>, dim3(4096), dim3(256), 0, c->stream,
+                         (int64_t)nblk, c->blocks64, c->blocks32);
+    }
+    if (c->layout == NH_LAYOUT_SK && !c->sk32) {
+      const int64_t ns = (int64_t)c->n_cations * 4 * kSkStride;
+      NH_HIP(hipMalloc(&c->sk32, ns * sizeof(float)));
+      hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
+                         ns, c->sk64, c->sk32);
+    }
+    NH_HIP(hipGetLastError());
+    NH_HIP(hipStreamSynchronize(c->stream));
+  }
+  c->precision = precision;
+  return NH_OK;
+}
+
 }  // namespace nh

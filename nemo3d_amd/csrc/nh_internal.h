@@ -98,6 +98,7 @@ struct nh_context {
 namespace nh {
 // kernels / launchers (nh_assemble.hip, nh_hv.hip, nh_vector.hip)
 nh_status assemble(nh_context *c, int layout, int precision);
+nh_status select_precision(nh_context *c, int precision);
 nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x,
                     nh_cplx *y, int ncol, int64_t ld);
 // block H·V of 8 vectors on the matrix cores (FP32 builds; nh_hv_mfma.hip)

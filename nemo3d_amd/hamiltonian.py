@@ -69,6 +69,11 @@ class Hamiltonian:
         native.check(native.load().nh_ham_build(self._ctx, layout, precision))
         self.layout, self.precision = layout, precision
 
+    def set_precision(self, precision):
+        """Arithmetic of later products (FP64 values are always kept)."""
+        native.check(native.load().nh_ham_set_precision(self._ctx, int(precision)))
+        self.precision = precision
+
     def set(self, what, value):
         """Kernel variant switch (nh_context_set)."""
         native.check(native.load().nh_context_set(self._ctx, int(what), int(value)))

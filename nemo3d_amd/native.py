@@ -126,6 +126,7 @@ SIGNATURES = {
     "nh_context_create": (ctypes.c_int, [P, ctypes.c_int, P]),
     "nh_context_destroy": (None, [P]),
     "nh_context_set": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+    "nh_ham_set_precision": (ctypes.c_int, [P, ctypes.c_int]),
     "nh_context_dim": (ctypes.c_int64, [P]),
     "nh_ham_build": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
     "nh_hv": (ctypes.c_int, [P, ctypes.c_double, P, P, ctypes.c_int]),

@@ -160,3 +160,53 @@ def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
     if vec is not None:
         out["vectors"] = vec[:n]
     return out
+
+
+def eigensolve_mixed(H, block_size=8, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
+                     erange_cb=(1.4, 2.0), tol=1e-8, tol32=1e-6,
+                     max_iter32=2000, max_iter64=400, seed=1, start_block=None,
+                     stop_rule="window", return_vectors=False, capacity=64):
+    """Mixed-precision block Lanczos: the bulk iterations with FP32 H·V
+    (FP32 values, FP32 products inside each block; on the matrix cores for
+    a block of 8) and FP64 recurrences, dots and Ritz solves, to an
+    FP32-level tolerance `tol32`; then FP64 block Lanczos started from
+    those Ritz vectors to `tol`, with FP64 Rayleigh quotients and residuals.
+    start_block: optional (block_size, N) start, e.g. the learned one."""
+    from . import native as nt
+    prec0 = H.precision
+    t_all = 0.0
+    try:
+        H.set_precision(nt.PREC_FP32)
+        r1 = eigensolve_block_lanczos(
+            H, block_size=block_size, num_vb=num_vb, num_cb=num_cb,
+            erange_vb=erange_vb, erange_cb=erange_cb, max_iter=max_iter32,
+            tol=tol32, start_iter=4, skip=4, seed=seed, start_block=start_block,
+            return_vectors=True, capacity=capacity, stop_rule=stop_rule)
+        t_all += r1["seconds"]
+        V = r1["vectors"]
+        if len(V) < block_size:
+            rng = np.random.default_rng(seed + 1)
+            extra = rng.random((block_size - len(V), H.n)) + 1j * rng.random(
+                (block_size - len(V), H.n))
+            V = np.concatenate([V, extra]) if len(V) else extra
+        V = V[:block_size]
+        H.set_precision(nt.PREC_FP64)
+        r2 = eigensolve_block_lanczos(
+            H, block_size=block_size, num_vb=num_vb, num_cb=num_cb,
+            erange_vb=erange_vb, erange_cb=erange_cb, max_iter=max_iter64,
+            tol=tol, start_iter=1, skip=1, start_block=V,
+            return_vectors=return_vectors, capacity=capacity,
+            stop_rule=stop_rule)
+        t_all += r2["seconds"]
+    finally:
+        if prec0 is not None:
+            H.set_precision(prec0)
+    out = dict(r2)
+    out["seconds"] = t_all
+    out["stage32"] = {k: r1[k] for k in ("iterations", "matvecs", "seconds",
+                                          "converged")}
+    out["stage32"]["eigenvalues"] = r1["eigenvalues"]
+    out["stage64"] = {k: r2[k] for k in ("iterations", "matvecs", "seconds",
+                                          "converged")}
+    out["matvecs"] = r1["matvecs"] + r2["matvecs"]
+    return out

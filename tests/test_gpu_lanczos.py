@@ -128,3 +128,23 @@ def test_tracemin_matches_dense():
     assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
     for e, v in zip(r["eigenvalues"], r["vectors"]):
         assert np.linalg.norm(ref.hv(v) - e * v) < 1e-7
+
+
+def test_mixed_precision_matches_dense():
+    """FP32 block Lanczos + FP64 refinement: eigenvalues to the FP64 bar."""
+    from nemo3d_amd.solvers import eigensolve_mixed
+    M = _models.gaas_box(cells=(3, 2, 2), strain_nm=0.01, seed=33)
+    ref = oracle.assemble(M.arrays())
+    w = distinct(np.linalg.eigvalsh(ref.dense()))
+    wv, wc = gap_windows(w)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    r = eigensolve_mixed(H, num_vb=3, num_cb=3, erange_vb=wv, erange_cb=wc,
+                         tol=1e-8, tol32=1e-5, max_iter32=220, max_iter64=200,
+                         stop_rule="nearest", return_vectors=True)
+    assert r["converged"], r
+    want = np.sort(np.concatenate([w[(w > wv[0]) & (w <= wv[1])][-3:],
+                                   w[(w > wc[0]) & (w <= wc[1])][:3]]))
+    assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
+    for e, v in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(v) - e * v) < 1e-6
+    assert H.precision == native.PREC_FP64
