@@ -32,8 +32,12 @@ class Tracemin {
 
  private:
   nh_status alloc(double2 **q, int cols) {
-    NH_HIP(hipMalloc(q, (size_t)cols * n_ * sizeof(double2)));
-    mem_.push_back(*q);
+    const size_t bytes = (size_t)cols * (size_t)n_ * sizeof(double2);
+    void *p = nullptr;
+    NH_HIP(hipMalloc(&p, bytes));
+    NH_CHECK(p, NH_ERR_NOMEM, "TRACEMIN: allocation of %zu bytes failed", bytes);
+    mem_.push_back(p);
+    *q = static_cast<double2 *>(p);
     return NH_OK;
   }
   int grid() const {

@@ -78,38 +78,72 @@ def test_lanczos_window_rule():
     assert (r["residuals"] < 1e-6).all()
 
 
+def tight_windows(w, nv, nc):
+    """Windows holding exactly the nv highest valence and nc lowest
+    conduction levels of the distinct spectrum w."""
+    sel = np.nonzero((w > -1.0) & (w < 3.0))[0]
+    k = sel[np.argmax(np.diff(w[sel]))]  # w[k]: top of the valence band
+    vb, cb = w[k - nv + 1:k + 1], w[k + 1:k + 1 + nc]
+    wv = (0.5 * (w[k - nv] + w[k - nv + 1]), w[k] + 1e-3)
+    wc = (w[k + 1] - 1e-3, 0.5 * (w[k + nc] + w[k + nc + 1]))
+    return vb, cb, wv, wc
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("case", ["si10", "gaas20"])
 def test_block_lanczos_matches_dense(case, precision):
+    """Block Lanczos (block 8) with the reference window rule on windows
+    that hold exactly the wanted levels.  The boxes are large enough that
+    the block Krylov space does not exhaust the space (N / 8 steps) before
+    convergence; FP32 products converge to an FP32-level tolerance."""
     from nemo3d_amd.solvers import eigensolve_block_lanczos
-    # boxes large enough that the block Krylov space (8 vectors a step)
-    # does not exhaust the space before convergence
     if case == "si10":
         M = _models.si_box(cells=(3, 3, 3), strain_nm=0.01, seed=31)
+        nv = nc = 4
     else:
         M = _models.gaas_box(cells=(3, 2, 2), strain_nm=0.01, seed=32)
+        nv = nc = 2
     print("case", case, precision, flush=True)
     ref = oracle.assemble(M.arrays())
-    wall = np.linalg.eigvalsh(ref.dense())
-    w = distinct(wall)
-    wv, wc = gap_windows(w)
+    w = distinct(np.linalg.eigvalsh(ref.dense()))
+    vb, cb, wv, wc = tight_windows(w, nv, nc)
     H = Hamiltonian(M, layout=native.LAYOUT_SK,
                     precision=native.PREC_FP64 if precision == "fp64"
                     else native.PREC_FP32)
-    r = eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
-                                 erange_vb=wv, erange_cb=wc, max_iter=220,
+    r = eigensolve_block_lanczos(H, block_size=8, num_vb=nv, num_cb=nc,
+                                 erange_vb=wv, erange_cb=wc,
+                                 max_iter=M.n // 8 - 10,
                                  tol=1e-8 if precision == "fp64" else 1e-6,
                                  start_iter=5, skip=5, return_vectors=True,
-                                 stop_rule="nearest")
+                                 stop_rule="window")
     assert r["converged"], r
     got = np.sort(r["eigenvalues"])
-    # distinct levels: a Kramers pair (20-band) counts once
-    want = np.sort(np.concatenate([w[(w > wv[0]) & (w <= wv[1])][-4:],
-                                   w[(w > wc[0]) & (w <= wc[1])][:4]]))
+    want = np.sort(np.concatenate([vb, cb]))
     tol_e = 1e-9 if precision == "fp64" else 1e-5
     assert np.abs(got - want).max() < tol_e, (got, want)
     for e, v in zip(r["eigenvalues"], r["vectors"]):
         assert np.linalg.norm(ref.hv(v) - e * v) < (1e-6 if precision == "fp64" else 1e-4)
+
+
+def test_block_lanczos_extreme_levels():
+    """The lowest levels of the spectrum (fast Krylov convergence): the
+    recurrence, Ritz extraction and eigenvector pass on a 20-band box,
+    nearest rule with the window's lower end acting as the gap edge."""
+    from nemo3d_amd.solvers import eigensolve_block_lanczos
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=34)
+    ref = oracle.assemble(M.arrays())
+    w = distinct(np.linalg.eigvalsh(ref.dense()))
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    r = eigensolve_block_lanczos(H, block_size=8, num_vb=0, num_cb=4,
+                                 erange_vb=(0.0, 0.0),
+                                 erange_cb=(w[0] - 1.0, w[6] + 1e-3),
+                                 max_iter=120, tol=1e-9, start_iter=5, skip=5,
+                                 return_vectors=True, stop_rule="nearest")
+    assert r["converged"], r
+    want = w[:4]
+    assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
+    for e, v in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(v) - e * v) < 1e-6
 
 
 def test_tracemin_matches_dense():
@@ -133,17 +167,17 @@ def test_tracemin_matches_dense():
 def test_mixed_precision_matches_dense():
     """FP32 block Lanczos + FP64 refinement: eigenvalues to the FP64 bar."""
     from nemo3d_amd.solvers import eigensolve_mixed
-    M = _models.gaas_box(cells=(3, 2, 2), strain_nm=0.01, seed=33)
+    M = _models.si_box(cells=(3, 3, 3), strain_nm=0.01, seed=33)
     ref = oracle.assemble(M.arrays())
     w = distinct(np.linalg.eigvalsh(ref.dense()))
-    wv, wc = gap_windows(w)
     H = Hamiltonian(M, layout=native.LAYOUT_SK)
-    r = eigensolve_mixed(H, num_vb=3, num_cb=3, erange_vb=wv, erange_cb=wc,
-                         tol=1e-8, tol32=1e-5, max_iter32=220, max_iter64=200,
-                         stop_rule="nearest", return_vectors=True)
+    vb, cb, wv, wc = tight_windows(w, 2, 2)
+    r = eigensolve_mixed(H, num_vb=2, num_cb=2, erange_vb=wv, erange_cb=wc,
+                         tol=1e-8, tol32=1e-5, max_iter32=M.n // 8 - 10,
+                         max_iter64=M.n // 8 - 10, stop_rule="window",
+                         return_vectors=True)
     assert r["converged"], r
-    want = np.sort(np.concatenate([w[(w > wv[0]) & (w <= wv[1])][-3:],
-                                   w[(w > wc[0]) & (w <= wc[1])][:3]]))
+    want = np.sort(np.concatenate([vb, cb]))
     assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
     for e, v in zip(r["eigenvalues"], r["vectors"]):
         assert np.linalg.norm(ref.hv(v) - e * v) < 1e-6
