@@ -1,0 +1,57 @@
+"""nd_* file IO against the reference's own files."""
+import os
+
+import numpy as np
+import pytest
+
+from nemo3d_amd import ndio
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_ascii_conversion_matches_reference_layout(tmp_path):
+    """Binary nd_Ek -> ASCII: header identical to the reference's converted
+    file (nemo3d/example/TestRun/InGaAs_QD.old_nd_Ek_ascii, same header),
+    records as %21.12e columns that parse back to the binary values."""
+    out = ndio.to_ascii(os.path.join(GOLDEN, "InGaAs_QD.nd_Ek"),
+                        str(tmp_path / "ek_ascii"))
+    got = open(out).read().split("\n")
+    want = open(os.path.join(GOLDEN, "InGaAs_QD.old_nd_Ek_ascii")).read().split("\n")
+    assert got[:5] == want[:5]
+    E = ndio.read_ek(os.path.join(GOLDEN, "InGaAs_QD.nd_Ek"))
+    rows = [l for l in got[5:] if l]
+    assert len(rows) == len(E) == 2
+    for line, e, ref_line in zip(rows, E, [l for l in want[5:] if l]):
+        assert len(line) == len(ref_line) == 84
+        vals = [float(line[i:i + 21]) for i in range(0, 84, 21)]
+        assert vals[:3] == [0.0, 0.0, 0.0] and abs(vals[3] - e) <= 5e-13 * abs(e)
+
+
+def test_writers_round_trip(tmp_path):
+    rng = np.random.default_rng(0)
+    E = rng.normal(size=7)
+    ndio.write_ek(str(tmp_path / "a.nd_Ek"), E)
+    assert np.array_equal(ndio.read_ek(str(tmp_path / "a.nd_Ek")), E)
+    psi = rng.normal(size=60) + 1j * rng.normal(size=60)
+    ndio.write_evec(str(tmp_path / "a.nd_evec_1"), psi, 20, "Bands_20_sp3d5ss_spin")
+    nd = ndio.read(str(tmp_path / "a.nd_evec_1"))
+    assert nd.dims == [(20, "# order of orbital basis"), (3, "# number of atoms")]
+    assert np.array_equal(ndio.read_evec(str(tmp_path / "a.nd_evec_1")), psi)
+    nbr = rng.integers(-1, 100, size=(5, 4))
+    ndio.write_nbrindx(str(tmp_path / "a.nd_nbrIndx"), nbr, "Bands_20_sp3d5ss_spin")
+    d = ndio.read(str(tmp_path / "a.nd_nbrIndx")).data
+    assert np.array_equal(np.stack([d[k] for k in ("n1", "n2", "n3", "n4")], 1), nbr)
+    with pytest.raises(ValueError):
+        ndio.write_ek(str(tmp_path / "b"), E.reshape(7, 1).repeat(2, 1))
+
+
+@pytest.mark.parametrize("name", ["silicon4.nd_rAtom", "silicon4.nd_nbrIndx",
+                                  "silicon4.nd_aType", "silicon4.nd_Ek"])
+def test_rewrite_reference_files_bytewise(name, tmp_path, reference_root):
+    """Reading a reference output and writing it back reproduces the file
+    byte for byte (header layout and record encoding)."""
+    src = os.path.join(reference_root, "PostNEMO3D/bin/test", name)
+    nd = ndio.read(src)
+    out = str(tmp_path / name)
+    ndio.write(out, nd.fmt, nd.dims, nd.user, nd.data)
+    assert open(out, "rb").read() == open(src, "rb").read()
