@@ -70,6 +70,7 @@ class BlockLanczos {
   BandT T_;
   std::vector<cd> Bnext_;
   std::vector<Factors> qr_;  // Cholesky QR factors of every step
+  KernelTimer timer_;        // sampled block H·V launches
   std::vector<Pair> conv_;
   std::vector<double> lastRitz_[2];
   std::vector<char> lastConv_[2];
@@ -169,9 +170,12 @@ nh_status BlockLanczos::replay_qr(double2 *&W, double2 *&Qout,
 
 nh_status BlockLanczos::step(int j, bool first_pass) {
   const int p = p_;
-  // W = H Q_j
+  // W = H Q_j (a sample of the first-pass products is timed)
+  const bool timed = first_pass && j % 4 == 1;
+  if (timed) NH_TRY(timer_.begin(c_->stream));
   NH_TRY(launch_hv(c_, 0.0, reinterpret_cast<const nh_cplx *>(Q_),
                    reinterpret_cast<nh_cplx *>(W_), p, n_));
+  if (timed) NH_TRY(timer_.end(c_->stream));
   if (j > 0) {  // W -= Q_{j-1} B_{j-1}^H
     const auto &B = T_.B[j - 1];
     std::vector<cd> M(p * p);
@@ -447,6 +451,9 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
   NH_HIP(hipStreamSynchronize(c->stream));
   res->matvecs = matvecs;
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  NH_TRY(timer_.collect(c->stream));
+  res->hv_ms = timer_.mean_ms();
+  res->hv_timed = timer_.count;
   return done ? NH_OK : NH_ERR_NOCONV;
 }
 

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/nemohip.h"
 
@@ -96,6 +98,51 @@ struct nh_context {
 };
 
 namespace nh {
+// Sampled kernel timing with HIP events on the launching stream: begin()/
+// end() around a launch; collect() after a synchronisation folds the
+// elapsed times into the mean.
+struct KernelTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  double sum_ms = 0.0;
+  int count = 0;
+  hipEvent_t pending = nullptr;
+  ~KernelTimer() { release(); }
+  void release() {
+    for (auto &e : ev) {
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
+    ev.clear();
+    if (pending) hipEventDestroy(pending);
+    pending = nullptr;
+  }
+  nh_status begin(hipStream_t s) {
+    NH_HIP(hipEventCreate(&pending));
+    NH_HIP(hipEventRecord(pending, s));
+    return NH_OK;
+  }
+  nh_status end(hipStream_t s) {
+    hipEvent_t e;
+    NH_HIP(hipEventCreate(&e));
+    NH_HIP(hipEventRecord(e, s));
+    ev.push_back({pending, e});
+    pending = nullptr;
+    return NH_OK;
+  }
+  nh_status collect(hipStream_t s) {
+    NH_HIP(hipStreamSynchronize(s));
+    for (auto &e : ev) {
+      float ms = 0.f;
+      NH_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+      sum_ms += ms;
+      ++count;
+    }
+    release();
+    return NH_OK;
+  }
+  double mean_ms() const { return count ? sum_ms / count : 0.0; }
+};
+
 // kernels / launchers (nh_assemble.hip, nh_hv.hip, nh_vector.hip)
 nh_status assemble(nh_context *c, int layout, int precision);
 nh_status select_precision(nh_context *c, int precision);

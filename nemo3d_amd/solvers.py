@@ -116,7 +116,8 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
     n = r.n_found
     out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
            "iterations": r.iterations, "matvecs": r.matvecs,
-           "seconds": r.seconds, "converged": st == native.NH_OK}
+           "seconds": r.seconds, "converged": st == native.NH_OK,
+           "hv_ms": r.hv_ms, "hv_timed": r.hv_timed}
     if vec is not None:
         out["vectors"] = vec[:n]
     return out
@@ -204,9 +205,9 @@ def eigensolve_mixed(H, block_size=8, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     out = dict(r2)
     out["seconds"] = t_all
     out["stage32"] = {k: r1[k] for k in ("iterations", "matvecs", "seconds",
-                                          "converged")}
+                                          "converged", "hv_ms", "hv_timed")}
     out["stage32"]["eigenvalues"] = r1["eigenvalues"]
     out["stage64"] = {k: r2[k] for k in ("iterations", "matvecs", "seconds",
-                                          "converged")}
+                                          "converged", "hv_ms", "hv_timed")}
     out["matvecs"] = r1["matvecs"] + r2["matvecs"]
     return out
