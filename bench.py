@@ -1,15 +1,29 @@
-"""Benchmark: time to 8 eigenpairs near the gap of an InAs/GaAs quantum dot.
+"""Benchmark: seconds to 8 eigenpairs near the gap of an InAs/GaAs quantum
+dot, mixed-precision + learned warm start against the plain FP64 solver.
 
-Workload (default): lens/dome InAs dot, base 20 nm, height 5 nm, on a
-1-monolayer InAs wetting layer inside a 30 x 30 x 20 nm GaAs box
-(786,520 atoms), sp3d5s* with spin-orbit (20 orbitals per atom,
-N = 15.7M complex Hermitian), dangling-bond passivation 30 eV.  A step is
-one eigen-solve on the GPU to 4 valence + 4 conduction eigenpairs at a
-relative residual ||H psi - E psi|| / (||H|| ||psi||) <= 1e-8, eigenvectors
-included.  Inputs (structure, Hamiltonian) are built and resident in HBM
-before the timed region.
+Workload: lens/dome InAs dot, base 20 nm, height 5 nm, on a 1-monolayer InAs
+wetting layer inside a 30 x 30 x 20 nm GaAs box (786,520 atoms), positions
+relaxed by the package's Keating VFF, sp3d5s* with spin-orbit (20 orbitals
+per atom, N = 15.7M complex Hermitian), dangling-bond passivation 30 eV.
+The 8 eigenpairs are the 4 highest valence and the 4 lowest conduction
+levels (Erange windows below), each to a relative residual
+||H psi - E psi|| / (||H|| ||psi||) <= 1e-8 with ||H|| = 40 eV, eigenvectors
+included.  The structure and Hamiltonian are built and resident in HBM
+before the timed region; a step is one complete solve.
 
-Prints one JSON line (see the repository's bench contract).
+Forms:
+  plain  the reference algorithm: Lanczos without reorthogonalisation,
+         CyclicRandom start, window convergence, second pass for the
+         eigenvectors, FP64 products (SK-compressed layout) and FP64 vectors;
+  mixed  block Lanczos (block 8): FP32 products (block H·V on the matrix
+         cores) with FP64 recurrences to 1e-6, then FP64 block Lanczos from
+         those Ritz vectors to 1e-8; started from the learned block when a
+         trained network is available (nemo3d_amd/data/warmstart.npz),
+         otherwise from a random block.
+`value` is the time of the mixed solve; the plain time, the speed-up and
+every eigenvalue's difference to the plain FP64 result are in `config`.
+
+Prints one JSON line (the repository's bench contract).
 """
 import argparse
 import json
@@ -22,42 +36,47 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-from nemo3d_amd import materials, model, native, structure, vff  # noqa: E402
+from nemo3d_amd import native, vff, warmstart  # noqa: E402
 from nemo3d_amd.hamiltonian import Hamiltonian  # noqa: E402
-from nemo3d_amd.solvers import eigensolve_lanczos  # noqa: E402
+from nemo3d_amd.solvers import eigensolve_lanczos, eigensolve_mixed  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-RESID_REL = 1e-8
+HNORM = 40.0   # spectral radius of this H (Ritz extremes -12.4 / 39.4 eV)
+WINDOWS = dict(erange_vb=(0.60, 0.70), erange_cb=(1.45, 1.52))
+WEIGHTS = os.path.join(HERE, "nemo3d_amd", "data", "warmstart.npz")
 
 
-def build_dot(relax=True):
-    a = 0.56532
-    s = structure.dot_structure((30, 30, 20), (a, a, a), shape="dome",
-                                base_nm=20.0, height_nm=5.0, wl_ml=1)
-    M = model.TBModel(s, ["GaAs", "InAs"], materials.BANDS_20_SPIN,
-                      passivate=True, db_shift=30.0)
-    info = {}
-    if relax:
-        info = vff.relax(M)
+def build_dot():
+    M = warmstart.dot_model((30.0, 30.0, 20.0), 20.0, 5.0, 1.0, 0)
+    info = vff.relax(M)
     return M, info
 
 
-def hv_bytes_per_launch(M, precision):
-    """Algorithmic bytes of one fused Lanczos H·v launch in the stored-block
-    layout: every stored value once (onsite record, one 10x10 block per
-    present neighbour, neighbour index), x read once, q and y written once."""
+def sk_lanczos_bytes(M, vec_bytes=16):
+    """Algorithmic bytes of one fused Lanczos H·v launch in the SK layout:
+    onsite record (18 doubles) per atom, one 24-double record per bond
+    (each bond once), neighbour and bond indices (4 + 4 int32 per atom), the
+    atom lists, x read once, q and y written once."""
     A = M.n_atoms
-    nbonds = int((M.structure.nbr >= 0).sum())
-    vb = 8 if precision == native.PREC_FP64 else 4
-    mat = A * 18 * 8 + nbonds * 100 * vb + A * 4 * 4
-    vec = 3 * M.n * 16
-    return mat + vec
+    nb = int((M.structure.nbr >= 0).sum()) // 2
+    mat = A * 18 * 8 + nb * 24 * 8 + A * 4 * 4 * 2 + A * 4
+    return mat + 3 * M.n * vec_bytes
+
+
+def sk_block_bytes(M, ncol=8):
+    """Algorithmic bytes of one block H·V launch (8 vectors) in the SK
+    layout, FP32 records: onsite (FP64) per atom, 24 FP32 values per bond
+    (each bond once), indices, X read once and Y written once (FP64)."""
+    A = M.n_atoms
+    nb = int((M.structure.nbr >= 0).sum()) // 2
+    mat = A * 18 * 8 + nb * 24 * 4 + A * 4 * 4 * 2 + A * 4
+    return mat + 2 * ncol * M.n * 16
 
 
 def cpu_baseline(M, matvecs, sample_iters=2):
-    """The CPU oracle (1 core) on a bounded sample: `sample_iters` Lanczos
-    iterations of the same Hamiltonian, scaled to the GPU solve's H·v
-    count."""
+    """The oracle (C restatement, 1 core) on a bounded sample of the plain
+    solve: assembly plus `sample_iters` Lanczos iterations of the same
+    Hamiltonian, scaled to the plain solve's H·v count."""
     from oracle import oracle
     t0 = time.time()
     Ho = oracle.assemble(M.arrays())
@@ -70,27 +89,43 @@ def cpu_baseline(M, matvecs, sample_iters=2):
             "kind": "port",
             "sample": "oracle C restatement, 1 thread: assembly (%.1f s) + %d "
                       "Lanczos iterations timed (%.2f s/iteration), scaled to "
-                      "the GPU solve's %d H*v" % (t_asm, sample_iters, per_it,
-                                                 matvecs)}
+                      "the plain GPU solve's %d H*v" % (t_asm, sample_iters,
+                                                       per_it, matvecs)}
 
 
-def traffic_from_profiles():
-    path = os.path.join(HERE, "profiles", "hv_traffic.json")
+def traffic_from_profiles(name):
+    path = os.path.join(HERE, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        return json.load(f).get(name)
+
+
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
+
+
+def plain_solve(H, max_iter):
+    return eigensolve_lanczos(H, num_vb=4, num_cb=4, max_iter=max_iter, tol=1e-8,
+                              start_iter=200, skip=100, stop_rule="window",
+                              **WINDOWS)
+
+
+def mixed_solve(H, start):
+    return eigensolve_mixed(H, block_size=8, num_vb=4, num_cb=4, tol=1e-8,
+                            tol32=1e-6, max_iter32=3000, max_iter64=600,
+                            start_block=start, stop_rule="window", **WINDOWS)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--no-relax", action="store_true")
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--form", choices=["both", "plain", "mixed"], default="both")
+    ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--max-iter", type=int, default=30000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); "
@@ -98,36 +133,96 @@ def main():
         sys.exit(2)
 
     t0 = time.time()
-    M, vinfo = build_dot(relax=not args.no_relax)
-    prec = native.PREC_FP64 if args.precision == "fp64" else native.PREC_FP32
-    H = Hamiltonian(M, precision=prec)
-    setup_s = time.time() - t0
-    print("# setup %.1f s: %d atoms, N=%d, vff=%s" % (setup_s, M.n_atoms, M.n,
-                                                      json.dumps(vinfo)),
-          file=sys.stderr, flush=True)
+    M, vinfo = build_dot()
+    H = Hamiltonian(M, layout=native.LAYOUT_SK, precision=native.PREC_FP64)
+    H.set_precision(native.PREC_FP32)  # FP32 copies made outside the timing
+    H.set_precision(native.PREC_FP64)
+    net = None
+    if args.form != "plain" and os.path.exists(args.weights):
+        net, _ = warmstart.load(H, args.weights)
+    # warm the kernels (module load, first launches) outside the timing
+    eigensolve_lanczos(H, num_vb=0, num_cb=1, erange_cb=(100, 101), max_iter=20,
+                       start_iter=1000, want_vectors=False)
+    log("setup %.1f s: %d atoms, N=%d, vff %d iterations" % (
+        time.time() - t0, M.n_atoms, M.n, vinfo.get("iterations", 0)))
 
-    opts = dict(num_vb=4, num_cb=4, erange_vb=(-0.4, 0.45),
-                erange_cb=(0.9, 1.8), max_iter=args.max_iter, tol=1e-8,
-                start_iter=200, skip=50)
-    for w in range(args.warmup):
-        r = eigensolve_lanczos(H, **opts)
-        print("# warmup %d: %.2f s, %d iterations" % (w, r["seconds"],
-                                                      r["iterations"]),
-              file=sys.stderr, flush=True)
-    times, res = [], None
-    for k in range(args.steps):
+    def run_plain():
         t = time.time()
-        res = eigensolve_lanczos(H, **opts)
-        times.append(time.time() - t)
-        print("# step %d: %.2f s, %d iterations, E=%s resid=%s" % (
-            k, times[-1], res["iterations"], np.round(res["eigenvalues"], 6),
-            res["residuals"]), file=sys.stderr, flush=True)
-    sec = float(np.mean(times))
-    hnorm = 40.0  # spectral radius of the sp3d5s* H (Ritz extremes -12.4 / 39.4 eV)
-    rel = (res["residuals"] / hnorm).tolist()
-    nbytes = hv_bytes_per_launch(M, prec)
-    achieved = nbytes / (res["hv_ms"] * 1e-3) / 1e9 if res["hv_ms"] > 0 else 0.0
-    traffic = traffic_from_profiles()
+        r = plain_solve(H, args.max_iter)
+        return time.time() - t, r
+
+    def run_mixed():
+        t = time.time()
+        start = warmstart.start_block(net, H) if net is not None else None
+        t_cnn = time.time() - t
+        r = mixed_solve(H, start)
+        r["cnn_s"] = t_cnn
+        return time.time() - t, r
+
+    plain = mixed = None
+    for w in range(args.warmup):
+        if args.form in ("both", "plain"):
+            run_plain()
+        if args.form in ("both", "mixed"):
+            run_mixed()
+    tp, tm = [], []
+    for k in range(args.steps):
+        if args.form in ("both", "plain"):
+            t, plain = run_plain()
+            tp.append(t)
+            log("plain step %d: %.2f s, %d iterations, E=%s" % (
+                k, t, plain["iterations"], np.round(plain["eigenvalues"], 7)))
+        if args.form in ("both", "mixed"):
+            t, mixed = run_mixed()
+            tm.append(t)
+            log("mixed step %d: %.2f s (cnn %.2f s), stages %s / %s, E=%s" % (
+                k, t, mixed["cnn_s"], mixed["stage32"]["iterations"],
+                mixed["stage64"]["iterations"], np.round(mixed["eigenvalues"], 7)))
+
+    headline = mixed if mixed is not None else plain
+    sec = float(np.mean(tm if mixed is not None else tp))
+    rel = (np.asarray(headline["residuals"]) / HNORM).tolist()
+    cfg = {
+        "workload": "InAs dome dot 20x5 nm + 1 ML WL in 30x30x20 nm GaAs, "
+                    "sp3d5s*+SO, %d atoms, N=%d, 4 VB + 4 CB levels @1e-8 "
+                    "rel. residual (||H|| = %g eV)" % (M.n_atoms, M.n, HNORM),
+        "positions": "VFF-relaxed (device Keating, %d CG iterations)"
+                     % vinfo.get("iterations", 0),
+        "windows_eV": WINDOWS,
+        "form": "mixed+learned" if (mixed is not None and net is not None)
+                else ("mixed (random start)" if mixed is not None else "plain"),
+        "eigenvalues_eV": np.round(headline["eigenvalues"], 9).tolist(),
+        "rel_residuals": rel,
+        "converged": bool(headline["converged"]),
+    }
+    if plain is not None:
+        cfg["plain_fp64_s"] = float(np.mean(tp))
+        cfg["plain_iterations"] = int(plain["iterations"])
+        cfg["plain_matvecs"] = int(plain["matvecs"])
+        cfg["plain_eigenvalues_eV"] = np.round(plain["eigenvalues"], 9).tolist()
+    if mixed is not None:
+        cfg["mixed_s"] = float(np.mean(tm))
+        cfg["mixed_stage32"] = {k: v for k, v in mixed["stage32"].items()
+                                if k != "eigenvalues"}
+        cfg["mixed_stage64"] = mixed["stage64"]
+        cfg["mixed_matvecs"] = int(mixed["matvecs"])
+        cfg["cnn_start_s"] = mixed["cnn_s"]
+        if plain is not None and len(plain["eigenvalues"]) == len(mixed["eigenvalues"]):
+            d = np.sort(mixed["eigenvalues"]) - np.sort(plain["eigenvalues"])
+            cfg["eig_diff_vs_plain_eV"] = d.tolist()
+            cfg["speedup_vs_plain"] = cfg["plain_fp64_s"] / cfg["mixed_s"]
+
+    if mixed is not None:
+        nbytes = sk_block_bytes(M)
+        kms = mixed["stage32"]["hv_ms"]
+        kname = "k_hV_mfma<2,true> (block H*V of 8 vectors, SK FP32, MFMA)"
+        tr_key = "k_hV_mfma"
+    else:
+        nbytes = sk_lanczos_bytes(M)
+        kms = plain["hv_ms"]
+        kname = "k_lz_hv_sk<2,double> (fused Lanczos H*v, SK FP64)"
+        tr_key = "k_lz_hv_sk"
+    achieved = nbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     line = {
         "metric": "s to 8 eigenpairs @1e-8 resid (mixed+learned vs FP64 plain); H·v HBM GB/s",
         "value": sec,
@@ -139,35 +234,32 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if prec == native.PREC_FP64 else "c64-stored/c128-vectors",
+        "dtype": "c64 products / c128 recurrences" if mixed is not None else "c128",
         "data": "synthetic",
-        "config": {
-            "workload": "InAs dome dot 20x5 nm + 1 ML WL in 30x30x20 nm GaAs, "
-                        "sp3d5s*+SO, %d atoms, N=%d" % (M.n_atoms, M.n),
-            "form": "plain Lanczos, stored blocks %s" % args.precision,
-            "positions": "VFF-relaxed (this package's GPU Keating)"
-                         if not args.no_relax else "ideal",
-            "eigenpairs": int(len(res["eigenvalues"])),
-            "converged": bool(res["converged"]),
-            "iterations": int(res["iterations"]),
-            "matvecs": int(res["matvecs"]),
-            "eigenvalues_eV": np.round(res["eigenvalues"], 9).tolist(),
-            "rel_residuals": rel,
-        },
+        "config": cfg,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_lz_hv (fused Lanczos H*v, stored blocks)",
+            "kernel": kname,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": traffic_from_profiles(tr_key),
             "bytes_per_launch": nbytes,
-            "launch_ms": res["hv_ms"],
+            "launch_ms": kms,
         },
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(M, res["matvecs"])
+    if plain is not None:
+        pb = sk_lanczos_bytes(M)
+        line["roofline_plain"] = {
+            "kernel": "k_lz_hv_sk<2,double> (fused Lanczos H*v, SK FP64)",
+            "achieved": pb / (plain["hv_ms"] * 1e-3) / 1e9 if plain["hv_ms"] > 0 else 0.0,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": pb,
+            "launch_ms": plain["hv_ms"],
+            "traffic": traffic_from_profiles("k_lz_hv_sk")}
+        line["roofline_plain"]["frac"] = line["roofline_plain"]["achieved"] / HBM_PEAK_GBS
+    if not args.no_cpu_baseline and plain is not None:
+        line["cpu_baseline"] = cpu_baseline(M, plain["matvecs"])
     print(json.dumps(line), flush=True)
 
 

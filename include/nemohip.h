@@ -78,8 +78,8 @@ void nh_context_destroy(nh_context *ctx);
 int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
 
 /* Kernel variants (chosen by measured throughput; both are exact to the
- * same rounding).  NH_SET_HV_STAGED: 1 (default) stages the neighbours'
- * vector entries of the stored-block H·v in LDS, 0 reads them from the
+ * same rounding).  NH_SET_HV_STAGED: 1 stages the neighbours' vector
+ * entries of the stored-block H·v in LDS, 0 (default) reads them from the
  * caches by every lane. */
 enum { NH_SET_HV_STAGED = 1 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);

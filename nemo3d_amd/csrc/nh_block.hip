@@ -217,7 +217,10 @@ void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
   T_.m = (int)T_.A.size();
   T_.p = p_;
   const double tol = o->conv_tolerance * hnorm;
-  const double resolution = 1e-9 * hnorm;
+  // copies of one level closer than this count once: FP64 products keep
+  // Kramers partners degenerate to ~1e-12, FP32 products split them by
+  // ~1e-9 relative (a loose tolerance signals the FP32 stage)
+  const double resolution = std::max(1e-9, 0.1 * o->conv_tolerance) * hnorm;
   std::vector<cd> work, x;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};
   const double *win[2] = {o->erange_vb, o->erange_cb};

@@ -17,10 +17,10 @@ void set_error(const char *fmt, ...);
 
 #define NH_HIP(call)                                                       \
   do {                                                                     \
-    hipError_t e_ = (call);                                                \
-    if (e_ != hipSuccess) {                                                \
+    const hipError_t nh_hip_status_ = (call);                              \
+    if (nh_hip_status_ != hipSuccess) {                                    \
       ::nh::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,          \
-                      hipGetErrorString(e_));                              \
+                      hipGetErrorString(nh_hip_status_));                  \
       return NH_ERR_HIP;                                                   \
     }                                                                      \
   } while (0)
@@ -33,10 +33,12 @@ void set_error(const char *fmt, ...);
     }                                                                      \
   } while (0)
 
+// (the local's name must not collide with any identifier used in `expr`,
+// e.g. a member s_, which it would shadow inside its own initialiser)
 #define NH_TRY(expr)                                                       \
   do {                                                                     \
-    nh_status s_ = (expr);                                                 \
-    if (s_ != NH_OK) return s_;                                            \
+    const nh_status nh_try_status_ = (expr);                               \
+    if (nh_try_status_ != NH_OK) return nh_try_status_;                    \
   } while (0)
 
 // Onsite record per atom: 10 diagonal energies, 6 surface s-p couplings,
@@ -75,7 +77,10 @@ struct nh_context {
   int32_t *rev_slot = nullptr;  // [A][4] slot of this atom in the nbr's list
 
   // kernel variants (nh_context_set)
-  bool hv_staged = true;  // stored-block H·v: neighbour x staged in LDS
+  // stored-block H·v: neighbour x staged in LDS (measured on the 786,520-
+  // atom dot: faster for the plain product, slower inside the fused Lanczos
+  // kernel, so off by default)
+  bool hv_staged = false;
 
   // assembled Hamiltonian
   int layout = -1;
