@@ -355,7 +355,11 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
              window_done(1, o->num_eig_cb, o->stop_rule);
     }
   }
-  if (!done) analyse(o, hnorm);
+  if (!done) {
+    analyse(o, hnorm);
+    done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+           window_done(1, o->num_eig_cb, o->stop_rule);
+  }
 
   std::vector<double> pick;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};

@@ -42,14 +42,37 @@ def training_params(k, seed0=7000):
                 x_in=float(rng.uniform(0.6, 1.0)), seed=seed0 + k)
 
 
-def plain_levels(H, n_vb=4, n_cb=4, erange_vb=(0.3, 0.75),
-                 erange_cb=(1.3, 1.7), tol=1e-8, max_iter=40000):
+def find_gap(H, iters=400, lo=-1.0, hi=3.0):
+    """Band-gap edges from the Ritz values of a short Lanczos run: the
+    largest gap between Ritz values in [lo, hi] (eV)."""
+    from scipy.linalg import eigh_tridiagonal
+    r = eigensolve_lanczos(H, num_vb=0, num_cb=1, erange_cb=(1e3, 1e3 + 1),
+                           max_iter=iters, start_iter=10 * iters,
+                           want_vectors=False, return_tridiag=True)
+    a, b = r["alpha"], r["beta"]
+    w = eigh_tridiagonal(a, b[:len(a) - 1], eigvals_only=True,
+                         select="v", select_range=(lo, hi))
+    k = int(np.argmax(np.diff(w)))
+    return float(w[k]), float(w[k + 1])
+
+
+def plain_levels(H, n_vb=4, n_cb=4, erange_vb=None, erange_cb=None, tol=1e-8,
+                 max_iter=40000, width=0.6):
     """The n_vb highest valence and n_cb lowest conduction levels of the
-    plain FP64 Lanczos solver with their eigenvectors (host)."""
+    plain FP64 Lanczos solver with their eigenvectors (host).  Without
+    windows, they end at the gap edges found by find_gap (the gap interior
+    stays outside, so no unconverged Ritz value wandering in the gap is
+    counted)."""
+    if erange_vb is None or erange_cb is None:
+        ev, ec = find_gap(H)
+        m = 0.25 * (ec - ev)
+        erange_vb = (ev - width, ev + m)
+        erange_cb = (ec - m, ec + width)
     r = eigensolve_lanczos(H, num_vb=n_vb, num_cb=n_cb, erange_vb=erange_vb,
                            erange_cb=erange_cb, max_iter=max_iter, tol=tol,
-                           start_iter=200, skip=100, stop_rule="nearest",
+                           start_iter=200, skip=50, stop_rule="nearest",
                            return_vectors=True, capacity=n_vb + n_cb)
+    r["windows"] = (erange_vb, erange_cb)
     return r
 
 
@@ -111,6 +134,13 @@ def start_block(net, H, n_states=learned.N_STATES):
     g = learned.GridMap(H.model.structure)
     f = learned.features(H.model, g)
     return net.start_block(f, g, n_states)
+
+
+def start_vector(block):
+    """A single Lanczos start vector from a start block: the sum of its
+    normalised columns."""
+    b = np.asarray(block)
+    return (b / np.linalg.norm(b, axis=1, keepdims=True)).sum(axis=0)
 
 
 def subspace_overlap(block, vectors, nb=20):
