@@ -35,6 +35,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
+os.environ.setdefault("NEMOHIP_PROGRESS", "1")  # progress lines on stderr
 
 from nemo3d_amd import native, vff, warmstart  # noqa: E402
 from nemo3d_amd.hamiltonian import Hamiltonian  # noqa: E402

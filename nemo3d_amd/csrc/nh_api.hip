@@ -1,5 +1,7 @@
 // C-ABI entry points of libnemohip (declared in include/nemohip.h).
+#include <chrono>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -8,6 +10,23 @@
 namespace nh {
 
 static thread_local char g_err[1024] = "";
+
+Progress::Progress() {
+  const char *e = std::getenv("NEMOHIP_PROGRESS");
+  on = e && e[0] == '1';
+  last = std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void Progress::tick(const char *what, int iter, int extra) {
+  if (!on) return;
+  const double now = std::chrono::duration<double>(
+                         std::chrono::steady_clock::now().time_since_epoch()).count();
+  if (now - last < 15.0) return;
+  last = now;
+  std::fprintf(stderr, "# %s: iteration %d (%d)\n", what, iter, extra);
+  std::fflush(stderr);
+}
 
 void set_error(const char *fmt, ...) {
   va_list ap;

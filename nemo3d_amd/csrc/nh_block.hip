@@ -332,9 +332,11 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
   int j = 0, skip = 0, matvecs = 0;
   bool done = false;
   double hnorm = 1.0;
+  Progress prog;
   while (!done && j < o->max_iter) {
     NH_TRY(step(j, true));
     ++j;
+    prog.tick("block lanczos", j, (int)conv_.size());
     ++skip;
     matvecs += p_;
     if (j > o->conv_check_start && skip >= o->conv_check_skip) {

@@ -6,19 +6,20 @@
 namespace nh {
 namespace {
 
-// SK-compressed layout (see nh_hv_sk.h).  Workgroups [0, nbc) take cation
-// atoms, the rest anion atoms.
+// SK-compressed layout (see nh_hv_sk.h); workgroup -> cation or anion block
+// by sk_block_of.
 template <int NSP, typename R>
 __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
-    const int32_t *__restrict__ anions, int nbc,
+    const int32_t *__restrict__ anions, int ppx,
     const double *__restrict__ onsite, const R *__restrict__ sk,
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
     double shift, const double2 *__restrict__ x, double2 *__restrict__ y,
     int64_t ld) {
   constexpr int NB = NSP * 10;
-  const bool cat = (int)blockIdx.x < nbc;
-  const int bl = cat ? blockIdx.x : blockIdx.x - nbc;
+  const SkBlock wb = sk_block_of(blockIdx.x, ppx);
+  const bool cat = wb.cat;
+  const int bl = wb.bl;
   const int li = bl * kSkAtomsPerBlock + threadIdx.x / 4;
   const int slot = threadIdx.x & 3;
   const bool active = li < (cat ? nc : na);
@@ -83,14 +84,13 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
            "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
   if (hV_mfma_supported(c, ncol)) return launch_hV_mfma(c, shift, x, y, ld);
   if (c->layout == NH_LAYOUT_SK) {
-    const int nbc = (c->n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
-    const int nba = (c->n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
-    const dim3 g(nbc + nba, ncol);
+    const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
+    const dim3 g(16 * ppx, ncol);
     const double2 *xd = reinterpret_cast<const double2 *>(x);
     double2 *yd = reinterpret_cast<double2 *>(y);
 #define SKL(NSP, R, SKP)                                                     \
   hipLaunchKernelGGL((k_hv_sk<NSP, R>), g, dim3(kSkThreads), 0, c->stream,   \
-                     c->n_cations, c->cations, c->n_anions, c->anions, nbc,  \
+                     c->n_cations, c->cations, c->n_anions, c->anions, ppx,  \
                      c->onsite, SKP, c->nbr, c->bond_of, shift, xd, yd, ld)
     if (c->precision == NH_PREC_FP64) {
       if (c->nb == 20) SKL(2, double, c->sk64);

@@ -23,6 +23,28 @@ namespace nh {
 constexpr int kSkThreads = 256;
 constexpr int kSkAtomsPerBlock = kSkThreads / 4;
 
+// Workgroup -> (role, block of the cation or anion list).  Cation block k
+// and anion block k cover the same cells (both lists are in atom order), so
+// they run back to back on the same XCD and share the neighbours' vector
+// entries in its L2: consecutive workgroups go to different XCDs (round
+// robin over 8), workgroup b runs on XCD b % 8, and XCD x takes the blocks
+// [x * ppx, (x + 1) * ppx), alternating cation and anion.  Grid: 16 * ppx,
+// ppx = ceil(max(cation blocks, anion blocks) / 8).
+struct SkBlock {
+  bool cat;
+  int bl;
+};
+__device__ __forceinline__ SkBlock sk_block_of(int b, int ppx) {
+  const int x = b & 7, s = b >> 3;
+  return {(s & 1) == 0, x * ppx + (s >> 1)};
+}
+inline int sk_pairs_per_xcd(int n_cations, int n_anions) {
+  const int nbc = (n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+  const int nba = (n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+  const int nbmax = nbc > nba ? nbc : nba;
+  return (nbmax + 7) / 8;
+}
+
 // Onsite contribution (diagonal - shift, surface s-p couplings, spin-orbit)
 // to row (s, o) of atom with vector base pointer xa.
 template <int NSP>

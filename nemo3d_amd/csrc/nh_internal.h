@@ -148,6 +148,15 @@ struct KernelTimer {
   double mean_ms() const { return count ? sum_ms / count : 0.0; }
 };
 
+// Progress lines on stderr for long solves (NEMOHIP_PROGRESS=1): at most
+// one every 15 s, so that a caller's log shows the solve is alive.
+struct Progress {
+  bool on = false;
+  double last = 0.0;
+  Progress();
+  void tick(const char *what, int iter, int extra);
+};
+
 // kernels / launchers (nh_assemble.hip, nh_hv.hip, nh_vector.hip)
 nh_status assemble(nh_context *c, int layout, int precision);
 nh_status select_precision(nh_context *c, int precision);

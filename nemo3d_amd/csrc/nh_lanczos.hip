@@ -92,15 +92,16 @@ __global__ __launch_bounds__(kThreads) void k_lz_hv(
 template <int NSP, typename R>
 __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
-    const int32_t *__restrict__ anions, int nbc,
+    const int32_t *__restrict__ anions, int ppx,
     const double *__restrict__ onsite, const R *__restrict__ sk,
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
     const double2 *__restrict__ Rv, double2 *__restrict__ Q,
     double2 *__restrict__ Y, const double *__restrict__ beta_prev,
     double *__restrict__ part) {
   constexpr int NB = NSP * 10;
-  const bool cat = (int)blockIdx.x < nbc;
-  const int bl = cat ? blockIdx.x : blockIdx.x - nbc;
+  const SkBlock wb = sk_block_of(blockIdx.x, ppx);
+  const bool cat = wb.cat;
+  const int bl = wb.bl;
   const int li = bl * kSkAtomsPerBlock + threadIdx.x / 4;
   const int slot = threadIdx.x & 3;
   const bool active = li < (cat ? nc : na);
@@ -354,10 +355,9 @@ nh_status LanczosRun::collect_timing() {
 
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
-  const int nbc = (c->n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
-  const int nba = (c->n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
+  const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
   const int gridA = c->layout == NH_LAYOUT_SK
-                        ? nbc + nba
+                        ? 16 * ppx
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   double2 *Q = ring_ + (int64_t)(k % kRing) * n_;
   double2 *Qp = k > 0 ? ring_ + (int64_t)((k - 1) % kRing) * n_ : nullptr;
@@ -384,7 +384,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #define LZ_SK(NSP, R, SKP)                                                      \
   hipLaunchKernelGGL((k_lz_hv_sk<NSP, R>), dim3(gridA), dim3(kSkThreads), 0,    \
                      c->stream, c->n_cations, c->cations, c->n_anions,          \
-                     c->anions, nbc, c->onsite, SKP, c->nbr, c->bond_of, R_, Q, \
+                     c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of, R_, Q, \
                      Y_, bprev, part)
   if (c->layout == NH_LAYOUT_SK) {
     if (c->precision == NH_PREC_FP64) {
@@ -544,9 +544,11 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   int m = 0, skip = 0;
   bool done = false;
   int matvecs = 0;
+  Progress prog;
   while (!done && m + 1 < o->max_iter) {
     NH_TRY(iterate(m, true));
     ++m;
+    prog.tick("lanczos", m, (int)conv_.size());
     ++matvecs;
     ++skip;
     if (o->conv_check_start < m && o->conv_check_skip <= skip) {
