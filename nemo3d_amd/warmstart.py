@@ -42,22 +42,27 @@ def training_params(k, seed0=7000):
                 x_in=float(rng.uniform(0.6, 1.0)), seed=seed0 + k)
 
 
-def find_gap(H, iters=400, lo=-1.0, hi=3.0):
-    """Band-gap edges from the Ritz values of a short Lanczos run: the
-    largest gap between Ritz values in [lo, hi] (eV)."""
+def find_gap(H, iters=2000, lo=-1.0, hi=3.0, tol=1e-6):
+    """Band-gap edges from a Lanczos run: the largest gap between its
+    converged Ritz values (trailing eigenvector component below tol) in
+    [lo, hi] eV.  Converged Ritz values are eigenvalues of H, so the gap
+    found is a gap of the spectrum."""
     from scipy.linalg import eigh_tridiagonal
     r = eigensolve_lanczos(H, num_vb=0, num_cb=1, erange_cb=(1e3, 1e3 + 1),
                            max_iter=iters, start_iter=10 * iters,
                            want_vectors=False, return_tridiag=True)
     a, b = r["alpha"], r["beta"]
-    w = eigh_tridiagonal(a, b[:len(a) - 1], eigvals_only=True,
-                         select="v", select_range=(lo, hi))
+    w, z = eigh_tridiagonal(a, b[:len(a) - 1], select="v",
+                            select_range=(lo, hi))
+    w = w[np.abs(z[-1]) < tol]
+    if len(w) < 2:
+        raise RuntimeError("find_gap: too few converged Ritz values")
     k = int(np.argmax(np.diff(w)))
     return float(w[k]), float(w[k + 1])
 
 
 def plain_levels(H, n_vb=4, n_cb=4, erange_vb=None, erange_cb=None, tol=1e-8,
-                 max_iter=40000, width=0.6):
+                 max_iter=20000, width=0.6):
     """The n_vb highest valence and n_cb lowest conduction levels of the
     plain FP64 Lanczos solver with their eigenvectors (host).  Without
     windows, they end at the gap edges found by find_gap (the gap interior
