@@ -15,11 +15,13 @@ Forms:
   plain  the reference algorithm: Lanczos without reorthogonalisation,
          CyclicRandom start, window convergence, second pass for the
          eigenvectors, FP64 products (SK-compressed layout) and FP64 vectors;
-  mixed  block Lanczos (block 8): FP32 products (block H·V on the matrix
-         cores) with FP64 recurrences to 1e-6, then FP64 block Lanczos from
-         those Ritz vectors to 1e-8; started from the learned block when a
-         trained network is available (nemo3d_amd/data/warmstart.npz),
-         otherwise from a random block.
+  mixed  the same Lanczos run and eigenvector pass with FP32 products (FP32
+         SK values and arithmetic inside each bond block, FP64 sums over
+         the bonds) while the recurrence, dots and tridiagonal stay FP64,
+         then FP64 residual refinement (block Krylov + Rayleigh-Ritz) of
+         its Ritz pairs to the 1e-8 residual; started from the learned
+         block (sum of its columns) when a trained network is available
+         (nemo3d_amd/data/warmstart.npz), otherwise CyclicRandom.
 `value` is the time of the mixed solve; the plain time, the speed-up and
 every eigenvalue's difference to the plain FP64 result are in `config`.
 
@@ -53,25 +55,16 @@ def build_dot():
     return M, info
 
 
-def sk_lanczos_bytes(M, vec_bytes=16):
+def sk_lanczos_bytes(M, vec_bytes=16, rec_bytes=8):
     """Algorithmic bytes of one fused Lanczos H·v launch in the SK layout:
-    onsite record (18 doubles) per atom, one 24-double record per bond
-    (each bond once), neighbour and bond indices (4 + 4 int32 per atom), the
-    atom lists, x read once, q and y written once."""
+    onsite record (18 doubles) per atom, one 24-value record per bond
+    (each bond once; rec_bytes per value), neighbour and bond indices
+    (4 + 4 int32 per atom), the atom lists, x read once, q and y written
+    once."""
     A = M.n_atoms
     nb = int((M.structure.nbr >= 0).sum()) // 2
-    mat = A * 18 * 8 + nb * 24 * 8 + A * 4 * 4 * 2 + A * 4
+    mat = A * 18 * 8 + nb * 24 * rec_bytes + A * 4 * 4 * 2 + A * 4
     return mat + 3 * M.n * vec_bytes
-
-
-def sk_block_bytes(M, ncol=8):
-    """Algorithmic bytes of one block H·V launch (8 vectors) in the SK
-    layout, FP32 records: onsite (FP64) per atom, 24 FP32 values per bond
-    (each bond once), indices, X read once and Y written once (FP64)."""
-    A = M.n_atoms
-    nb = int((M.structure.nbr >= 0).sum()) // 2
-    mat = A * 18 * 8 + nb * 24 * 4 + A * 4 * 4 * 2 + A * 4
-    return mat + 2 * ncol * M.n * 16
 
 
 def cpu_baseline(M, matvecs, sample_iters=2):
@@ -112,10 +105,11 @@ def plain_solve(H, max_iter):
                               **WINDOWS)
 
 
-def mixed_solve(H, start):
-    return eigensolve_mixed(H, block_size=8, num_vb=4, num_cb=4, tol=1e-8,
-                            tol32=1e-6, max_iter32=3000, max_iter64=600,
-                            start_block=start, stop_rule="window", **WINDOWS)
+def mixed_solve(H, start, max_iter):
+    return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, hnorm=HNORM,
+                            max_iter=max_iter, start_iter=200, skip=100,
+                            start_vector=start, stop_rule="window",
+                            refine_depth=12, refine_rounds=4, **WINDOWS)
 
 
 def main():
@@ -154,9 +148,10 @@ def main():
 
     def run_mixed():
         t = time.time()
-        start = warmstart.start_block(net, H) if net is not None else None
+        start = (warmstart.start_vector(warmstart.start_block(net, H))
+                 if net is not None else None)
         t_cnn = time.time() - t
-        r = mixed_solve(H, start)
+        r = mixed_solve(H, start, args.max_iter)
         r["cnn_s"] = t_cnn
         return time.time() - t, r
 
@@ -176,9 +171,11 @@ def main():
         if args.form in ("both", "mixed"):
             t, mixed = run_mixed()
             tm.append(t)
-            log("mixed step %d: %.2f s (cnn %.2f s), stages %s / %s, E=%s" % (
-                k, t, mixed["cnn_s"], mixed["stage32"]["iterations"],
-                mixed["stage64"]["iterations"], np.round(mixed["eigenvalues"], 7)))
+            log("mixed step %d: %.2f s (cnn %.2f s), FP32 stage %d iterations, "
+                "refinement %d rounds / %d H*v, E=%s" % (
+                    k, t, mixed["cnn_s"], mixed["stage32"]["iterations"],
+                    mixed["refine"]["rounds"], mixed["refine"]["matvecs"],
+                    np.round(mixed["eigenvalues"], 7)))
 
     headline = mixed if mixed is not None else plain
     sec = float(np.mean(tm if mixed is not None else tp))
@@ -204,8 +201,8 @@ def main():
     if mixed is not None:
         cfg["mixed_s"] = float(np.mean(tm))
         cfg["mixed_stage32"] = {k: v for k, v in mixed["stage32"].items()
-                                if k != "eigenvalues"}
-        cfg["mixed_stage64"] = mixed["stage64"]
+                                if k not in ("eigenvalues", "residuals")}
+        cfg["mixed_refine"] = mixed["refine"]
         cfg["mixed_matvecs"] = int(mixed["matvecs"])
         cfg["cnn_start_s"] = mixed["cnn_s"]
         if plain is not None and len(plain["eigenvalues"]) == len(mixed["eigenvalues"]):
@@ -214,10 +211,10 @@ def main():
             cfg["speedup_vs_plain"] = cfg["plain_fp64_s"] / cfg["mixed_s"]
 
     if mixed is not None:
-        nbytes = sk_block_bytes(M)
+        nbytes = sk_lanczos_bytes(M, rec_bytes=4)
         kms = mixed["stage32"]["hv_ms"]
-        kname = "k_hV_mfma<2,true> (block H*V of 8 vectors, SK FP32, MFMA)"
-        tr_key = "k_hV_mfma"
+        kname = "k_lz_hv_sk<2,float> (fused Lanczos H*v, SK FP32 products)"
+        tr_key = "k_lz_hv_sk_fp32"
     else:
         nbytes = sk_lanczos_bytes(M)
         kms = plain["hv_ms"]

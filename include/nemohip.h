@@ -239,6 +239,31 @@ typedef struct nh_tracemin_opts {
 nh_status nh_eigensolve_tracemin(nh_context *ctx, const nh_tracemin_opts *opts,
                                  nh_eig_result *res);
 
+/* FP64 refinement of approximate eigenpairs, the residual-refinement stage
+ * of the mixed-precision solver (Ritz pairs of a run with FP32 products
+ * are eigenpairs of the FP32-rounded operator).  Each round builds a block
+ * Krylov basis [X, H X, ..., H^depth X] (Arnoldi, classical Gram-Schmidt
+ * applied twice, Cholesky QR) and keeps, for every current eigenvalue, the
+ * nearest FP64 Ritz pair of the basis; rounds stop once every
+ * ||H x - e x|| <= tolerance * hnorm (||x|| = 1).  X [k][N] (host or
+ * device per ptr_kind, k <= 16) is overwritten with the refined vectors,
+ * eigenvalues (in: estimates) with the Ritz values, residuals (out) with
+ * ||H x - e x||.  The context must hold the FP64 operator.  No reference
+ * counterpart: the reference reports the residuals of its FP64 Ritz pairs
+ * (compute_ritz_residual_norm_deviation, nemo3d/src/base/h_cvectr_mult.c:
+ * 78-...) and refines nothing.  Returns NH_ERR_NOCONV when
+ * max_rounds end first (results still written). */
+typedef struct nh_refine_opts {
+  int32_t depth;       /* Krylov blocks per round, 1..32 */
+  int32_t max_rounds;
+  double tolerance;
+  double hnorm;
+} nh_refine_opts;
+nh_status nh_refine_eigenpairs(nh_context *ctx, const nh_refine_opts *opts,
+                               int32_t k, nh_cplx *X, int ptr_kind,
+                               double *eigenvalues, double *residuals,
+                               int32_t *rounds, int32_t *matvecs);
+
 /* ---- valence force field ------------------------------------------------
  * Keating VFF (calc_estrain / min_estrain, strain3d.c:157-475,1361-2245):
  * alpha, beta, a0 are [S][S] tables (Keating constants and unstrained
@@ -277,7 +302,7 @@ nh_status nh_cnn_forward(nh_cnn *cnn, const float *feat, int gx, int gy,
 nh_status nh_cnn_backward(nh_cnn *cnn, const float *feat, int gx, int gy,
                           int gz, const float *dout, float *grad);
 /* One Adam step on one structure of the context.  targets [n_targets][N]
- * (host, n_targets <= 16): eigenvectors of the plain solver, with their
+ * (host or device, n_targets <= 16): eigenvectors of the plain solver, with their
  * Kramers partners; cluster[n_targets] labels each target's degenerate
  * subspace and out_cluster[n_states] (n_states <= 8) the subspace each
  * predicted column should span.  Loss = mean over columns of

@@ -646,8 +646,9 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
   hipMemcpyAsync(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice, s);
   hipMemsetAsync(Y, 0, (size_t)8 * N * sizeof(double2), s);
   hipMemsetAsync(V, 0, (size_t)16 * N * sizeof(double2), s);
+  // targets may be host or device memory (unified addressing)
   hipMemcpyAsync(V, targets, (size_t)n_targets * N * sizeof(double2),
-                 hipMemcpyHostToDevice, s);
+                 hipMemcpyDefault, s);
   hipLaunchKernelGGL(k_map_out, dim3(g1(A * x->nb)), dim3(256), 0, s, A, x->nb,
                      ne, c->c_out, dp, ds, c->out, Y, N);
   // C = V^H Y (16 x 8, two 8x8 halves) and the Gram Y^H Y

@@ -196,4 +196,76 @@ inline void dense_eigh(int s, const std::vector<cd> &Hm, std::vector<double> &w,
   }
 }
 
+// R factor (cols x cols, row-major upper) of the Householder QR of A
+// (rows x cols, row-major, rows >= cols).
+inline std::vector<cd> qr_r(int rows, int cols, std::vector<cd> A) {
+  for (int j = 0; j < cols; ++j) {
+    double nrm = 0.0;
+    for (int r = j; r < rows; ++r) nrm += std::norm(A[r * cols + j]);
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) continue;
+    const cd a = A[j * cols + j];
+    const cd ph = std::abs(a) > 0.0 ? a / std::abs(a) : cd(1.0);
+    const cd alpha = -ph * nrm;  // R_jj; v = x - alpha e1
+    std::vector<cd> v(rows - j);
+    for (int r = j; r < rows; ++r) v[r - j] = A[r * cols + j];
+    v[0] -= alpha;
+    double vn = 0.0;
+    for (const auto &z : v) vn += std::norm(z);
+    if (vn == 0.0) continue;
+    for (int c = j; c < cols; ++c) {
+      cd d = 0.0;
+      for (int r = j; r < rows; ++r) d += std::conj(v[r - j]) * A[r * cols + c];
+      d *= 2.0 / vn;
+      for (int r = j; r < rows; ++r) A[r * cols + c] -= d * v[r - j];
+    }
+  }
+  std::vector<cd> R((size_t)cols * cols, 0.0);
+  for (int r = 0; r < cols; ++r)
+    for (int c = r; c < cols; ++c) R[(size_t)r * cols + c] = A[(size_t)r * cols + c];
+  return R;
+}
+
+// Refined Ritz vector of a Krylov basis B (m columns, orthonormal) with
+// H B = B T + W E^T, W orthogonal to B, W^H W = Rw^H Rw (Rw q x q upper)
+// and E^T selecting the last q coefficients: the unit y minimising
+// ||(H - theta) B y||^2 = ||(T - theta) y||^2 + ||Rw y_last||^2, by
+// inverse iteration on the R factor of [T - theta I; Rw E^T] starting from
+// y (in/out).  T: m x m row-major (the Arnoldi matrix, not symmetrised).
+inline void refined_ritz(int m, int q, const std::vector<cd> &T,
+                         const std::vector<cd> &Rw, double theta,
+                         std::vector<cd> &y, int iters = 3) {
+  const int rows = m + q;
+  std::vector<cd> A((size_t)rows * m, 0.0);
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c)
+      A[(size_t)r * m + c] = T[(size_t)r * m + c] - (r == c ? theta : 0.0);
+  for (int r = 0; r < q; ++r)
+    for (int c = 0; c < q; ++c) A[(size_t)(m + r) * m + m - q + c] = Rw[r * q + c];
+  std::vector<cd> R = qr_r(rows, m, A);
+  double dmax = 0.0;
+  for (int i = 0; i < m; ++i) dmax = std::max(dmax, std::abs(R[(size_t)i * m + i]));
+  const double floor_ = 1e-15 * std::max(dmax, 1e-300);
+  for (int i = 0; i < m; ++i)
+    if (std::abs(R[(size_t)i * m + i]) < floor_) R[(size_t)i * m + i] = floor_;
+  for (int it = 0; it < iters; ++it) {
+    // R^H z = y (lower), then R y' = z (upper)
+    std::vector<cd> z(m);
+    for (int i = 0; i < m; ++i) {
+      cd s = y[i];
+      for (int k = 0; k < i; ++k) s -= std::conj(R[(size_t)k * m + i]) * z[k];
+      z[i] = s / std::conj(R[(size_t)i * m + i]);
+    }
+    for (int i = m - 1; i >= 0; --i) {
+      cd s = z[i];
+      for (int k = i + 1; k < m; ++k) s -= R[(size_t)i * m + k] * y[k];
+      y[i] = s / R[(size_t)i * m + i];
+    }
+    double n = 0.0;
+    for (const auto &v : y) n += std::norm(v);
+    n = std::sqrt(n);
+    for (auto &v : y) v /= n;
+  }
+}
+
 }  // namespace nh

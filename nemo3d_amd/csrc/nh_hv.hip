@@ -26,18 +26,21 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
   const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
   const double2 *xc = x + (int64_t)blockIdx.y * ld;
   double2 *yc = y + (int64_t)blockIdx.y * ld;
-  double ar[NB], ai[NB];
+  constexpr int NO = (NB + 3) / 4;
+  double own_re[NO], own_im[NO];
   if (cat)
-    sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, xc, ar, ai);
+    sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, xc, own_re, own_im);
   else
-    sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, xc, ar, ai);
+    sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, xc, own_re, own_im);
   if (!active) return;
   const double *os = onsite + (size_t)atom * kOnsiteStride;
   const double2 *xa = xc + (int64_t)atom * NB;
+  // all four lanes of the quad active on every step (row r = 4 i + slot)
 #pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    if ((r & 3) != slot) continue;
-    double re = ar[r], im = ai[r];
+  for (int i = 0; i < NO; ++i) {
+    const int r = 4 * i + slot;
+    if (r >= NB) break;
+    double re = own_re[i], im = own_im[i];
     onsite_row<NSP>(os, xa, r / 10, r % 10, shift, re, im);
     yc[(int64_t)atom * NB + r] = make_double2(re, im);
   }

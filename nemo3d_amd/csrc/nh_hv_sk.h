@@ -132,54 +132,107 @@ struct BondApply {
   }
 };
 
-// Accumulates the 4 bonds of `atom` (lane = slot) into acc, summed over the
-// 4 lanes; returns with every lane of the quad holding the full sums.
-template <int NSP, typename R, bool CATION>
+// The 24 record values of a bond as 16-byte loads (12 for FP64, 6 for
+// FP32): the records are 16-byte aligned (24 values per bond).  Across a
+// wavefront every lane reads its own record, so the number of load
+// instructions, each touching 64 cache lines, is what the texture
+// addresser pays for.
+__device__ __forceinline__ void sk_load_record(const double *rec, double (&v)[24]) {
+  const double2 *p = reinterpret_cast<const double2 *>(rec);
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    const double2 t = p[q];
+    v[2 * q] = t.x;
+    v[2 * q + 1] = t.y;
+  }
+}
+__device__ __forceinline__ void sk_load_record(const float *rec, float (&v)[24]) {
+  const float4 *p = reinterpret_cast<const float4 *>(rec);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const float4 t = p[q];
+    v[4 * q] = t.x;
+    v[4 * q + 1] = t.y;
+    v[4 * q + 2] = t.z;
+    v[4 * q + 3] = t.w;
+  }
+}
+
+// The neighbour's NB complex entries, from the FP64 vector (one double2
+// per entry) or from its FP32 copy (one float4 per two entries; the entries
+// of an atom start on a 16-byte boundary, NB * 8 bytes per atom).
+template <int NSP, typename R>
+__device__ __forceinline__ void sk_load_x(const double2 *__restrict__ x,
+                                          int64_t j, R (&xr)[NSP][10],
+                                          R (&xi)[NSP][10]) {
+  const double2 *xj = x + j * (NSP * 10);
+#pragma unroll
+  for (int s = 0; s < NSP; ++s)
+#pragma unroll
+    for (int c = 0; c < 10; ++c) {
+      const double2 t = xj[s * 10 + c];
+      xr[s][c] = R(t.x);
+      xi[s][c] = R(t.y);
+    }
+}
+template <int NSP, typename R>
+__device__ __forceinline__ void sk_load_x(const float2 *__restrict__ x,
+                                          int64_t j, R (&xr)[NSP][10],
+                                          R (&xi)[NSP][10]) {
+  const float4 *xj = reinterpret_cast<const float4 *>(x + j * (NSP * 10));
+#pragma unroll
+  for (int s = 0; s < NSP; ++s)
+#pragma unroll
+    for (int c = 0; c < 10; c += 2) {
+      const float4 t = xj[(s * 10 + c) / 2];
+      xr[s][c] = R(t.x);
+      xi[s][c] = R(t.y);
+      xr[s][c + 1] = R(t.z);
+      xi[s][c + 1] = R(t.w);
+    }
+}
+
+// The 4 bonds of `atom` (lane = slot), summed over the quad in FP64 (fixed
+// order); lane `slot` keeps rows r = 4 i + slot in own_re[i] / own_im[i].
+// XT: double2 (the FP64 vector) or float2 (its FP32 copy, R = float).
+template <int NSP, typename R, bool CATION, typename XT>
 __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
                                          const R *__restrict__ sk,
                                          const int32_t *__restrict__ nbr,
                                          const int32_t *__restrict__ bond_of,
-                                         const double2 *__restrict__ xc,
-                                         double (&ar)[NSP * 10],
-                                         double (&ai)[NSP * 10]) {
-  constexpr int NB = NSP * 10;
+                                         const XT *__restrict__ xc,
+                                         double (&own_re)[(NSP * 10 + 3) / 4],
+                                         double (&own_im)[(NSP * 10 + 3) / 4]) {
+  R br[NSP][10], bi[NSP][10];
 #pragma unroll
-  for (int r = 0; r < NB; ++r) ar[r] = ai[r] = 0.0;
+  for (int s = 0; s < NSP; ++s)
+#pragma unroll
+    for (int c = 0; c < 10; ++c) br[s][c] = bi[s][c] = R(0);
   const int j = active ? nbr[atom * 4 + slot] : -1;
   if (j >= 0) {
-    const R *rec = sk + (size_t)bond_of[atom * 4 + slot] * kSkStride;
-    R v[21];
-#pragma unroll
-    for (int k = 0; k < 21; ++k) v[k] = rec[3 + k];
-    const R l = rec[0], m = rec[1], n = rec[2];
-    R xr[NSP][10], xi[NSP][10], br[NSP][10], bi[NSP][10];
-    const double2 *xj = xc + (int64_t)j * NB;
-#pragma unroll
-    for (int s = 0; s < NSP; ++s)
-#pragma unroll
-      for (int c = 0; c < 10; ++c) {
-        const double2 t = xj[s * 10 + c];
-        xr[s][c] = R(t.x);
-        xi[s][c] = R(t.y);
-        br[s][c] = bi[s][c] = R(0);
-      }
-    BondApply<NSP, R, CATION>::template run<0>(l, m, n, v, xr, xi, br, bi);
-#pragma unroll
-    for (int s = 0; s < NSP; ++s)
-#pragma unroll
-      for (int c = 0; c < 10; ++c) {
-        ar[s * 10 + c] = double(br[s][c]);
-        ai[s * 10 + c] = double(bi[s][c]);
-      }
+    R rv[24];
+    sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
+    R xr[NSP][10], xi[NSP][10];
+    sk_load_x<NSP, R>(xc, j, xr, xi);
+    BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3, xr,
+                                                xi, br, bi);
   }
-  // sum over the quad (lanes 4k..4k+3); fixed order, every lane gets it
+  // sum over the quad (lanes 4k..4k+3); the owner lane keeps the row
 #pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    ar[r] += __shfl_xor(ar[r], 1, 64);
-    ai[r] += __shfl_xor(ai[r], 1, 64);
-    ar[r] += __shfl_xor(ar[r], 2, 64);
-    ai[r] += __shfl_xor(ai[r], 2, 64);
-  }
+  for (int s = 0; s < NSP; ++s)
+#pragma unroll
+    for (int c = 0; c < 10; ++c) {
+      const int r = s * 10 + c;
+      double re = double(br[s][c]), im = double(bi[s][c]);
+      re += __shfl_xor(re, 1, 64);
+      im += __shfl_xor(im, 1, 64);
+      re += __shfl_xor(re, 2, 64);
+      im += __shfl_xor(im, 2, 64);
+      if ((r & 3) == slot) {
+        own_re[r >> 2] = re;
+        own_im[r >> 2] = im;
+      }
+    }
 }
 
 }  // namespace nh

@@ -95,10 +95,11 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
     const int32_t *__restrict__ anions, int ppx,
     const double *__restrict__ onsite, const R *__restrict__ sk,
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
-    const double2 *__restrict__ Rv, double2 *__restrict__ Q,
-    double2 *__restrict__ Y, const double *__restrict__ beta_prev,
-    double *__restrict__ part) {
+    const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
+    double2 *__restrict__ Q, double2 *__restrict__ Y,
+    const double *__restrict__ beta_prev, double *__restrict__ part) {
   constexpr int NB = NSP * 10;
+  constexpr int NO = (NB + 3) / 4;
   const SkBlock wb = sk_block_of(blockIdx.x, ppx);
   const bool cat = wb.cat;
   const int bl = wb.bl;
@@ -107,19 +108,29 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
   const bool active = li < (cat ? nc : na);
   const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
   const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
-  double ar[NB], ai[NB];
-  if (cat)
-    sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, Rv, ar, ai);
-  else
-    sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, Rv, ar, ai);
+  double own_re[NO], own_im[NO];
+  // FP32 products gather from the FP32 copy of r (the same rounding as
+  // converting the FP64 entries on load)
+  if constexpr (sizeof(R) == 4) {
+    if (cat)
+      sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, Rv32, own_re, own_im);
+    else
+      sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, Rv32, own_re, own_im);
+  } else {
+    if (cat)
+      sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, Rv, own_re, own_im);
+    else
+      sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, Rv, own_re, own_im);
+  }
   double local = 0.0;
   if (active) {
     const double *os = onsite + (size_t)atom * kOnsiteStride;
     const double2 *xa = Rv + (int64_t)atom * NB;
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      if ((r & 3) != slot) continue;
-      double re = ar[r], im = ai[r];
+    for (int i = 0; i < NO; ++i) {
+      const int r = 4 * i + slot;
+      if (r >= NB) break;
+      double re = own_re[i], im = own_im[i];
       onsite_row<NSP>(os, xa, r / 10, r % 10, 0.0, re, im);
       const double2 xr = xa[r];
       const double2 q = make_double2(xr.x * inv, xr.y * inv);
@@ -137,10 +148,12 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
 }
 
 // y <- y - alpha q - beta_prev q_prev, partial sums of |y|^2.
+// Y32 (optional): FP32 copy of the new r for the next FP32 product.
 __global__ __launch_bounds__(kRedThreads) void k_lz_update(
     int64_t n, double2 *__restrict__ Y, const double2 *__restrict__ Q,
     const double2 *__restrict__ Qp, const double *__restrict__ alpha,
-    const double *__restrict__ beta_prev, double *__restrict__ part) {
+    const double *__restrict__ beta_prev, double *__restrict__ part,
+    float2 *__restrict__ Y32) {
   const double al = alpha[0];
   const double be = beta_prev ? beta_prev[0] : 0.0;
   double s = 0.0, z = 0.0;
@@ -157,6 +170,7 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_update(
       y.y -= al * q.y;
     }
     Y[k] = y;
+    if (Y32) Y32[k] = make_float2((float)y.x, (float)y.y);
     s += y.x * y.x + y.y * y.y;
   }
   if (part) {
@@ -317,6 +331,9 @@ class LanczosRun {
   std::vector<void *> allocs_;
   int64_t n_ = 0;
   double2 *R_ = nullptr, *Y_ = nullptr, *ring_ = nullptr;
+  // FP32 copies of r for the FP32 SK products (null otherwise)
+  float2 *R32_ = nullptr, *Y32_ = nullptr;
+  nh_status upload_start(const std::vector<nh_cplx> &r0);
   double *alpha_ = nullptr, *beta_ = nullptr, *part_ = nullptr;
   int nparts_ = 0;
   std::vector<double> ha_, hb_;
@@ -353,6 +370,27 @@ nh_status LanczosRun::collect_timing() {
   return NH_OK;
 }
 
+__global__ __launch_bounds__(256) void k_to_f32(int64_t n,
+                                                const double2 *__restrict__ x,
+                                                float2 *__restrict__ y) {
+  for (int64_t k = blockIdx.x * (int64_t)256 + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * 256)
+    y[k] = make_float2((float)x[k].x, (float)x[k].y);
+}
+
+// r <- r0 (and its FP32 copy when the products are FP32)
+nh_status LanczosRun::upload_start(const std::vector<nh_cplx> &r0) {
+  NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
+                        hipMemcpyHostToDevice, c_->stream));
+  if (R32_) {
+    hipLaunchKernelGGL(k_to_f32, dim3(reduce_blocks(n_)), dim3(256), 0,
+                       c_->stream, n_, R_, R32_);
+    NH_HIP(hipGetLastError());
+  }
+  // the host vector r0 lives on: the copy may still be in flight
+  return NH_OK;
+}
+
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
   const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
@@ -384,8 +422,8 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #define LZ_SK(NSP, R, SKP)                                                      \
   hipLaunchKernelGGL((k_lz_hv_sk<NSP, R>), dim3(gridA), dim3(kSkThreads), 0,    \
                      c->stream, c->n_cations, c->cations, c->n_anions,          \
-                     c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of, R_, Q, \
-                     Y_, bprev, part)
+                     c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of, R_,    \
+                     R32_, Q, Y_, bprev, part)
   if (c->layout == NH_LAYOUT_SK) {
     if (c->precision == NH_PREC_FP64) {
       if (c->nb == 20) LZ_SK(2, double, c->sk64);
@@ -412,12 +450,13 @@ nh_status LanczosRun::iterate(int k, bool red) {
                        gridA, 1, part_, alpha_ + k, 0);
   const int nb = reduce_blocks(n_);
   hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
-                     n_, Y_, Q, Qp, alpha_ + k, bprev, part);
+                     n_, Y_, Q, Qp, alpha_ + k, bprev, part, Y32_);
   if (red)
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        nb, 2, part_, beta_ + k, 1);
   NH_HIP(hipGetLastError());
   std::swap(R_, Y_);  // new r is in Y; old r buffer becomes next y
+  std::swap(R32_, Y32_);
   return NH_OK;
 }
 
@@ -537,8 +576,11 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     r0[j].re /= s;
     r0[j].im /= s;
   }
-  NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
-                        hipMemcpyHostToDevice, c->stream));
+  if (c->layout == NH_LAYOUT_SK && c->precision == NH_PREC_FP32) {
+    NH_TRY(alloc((void **)&R32_, n_ * sizeof(float2)));
+    NH_TRY(alloc((void **)&Y32_, n_ * sizeof(float2)));
+  }
+  NH_TRY(upload_start(r0));
 
   // pass 1: eigenvalues
   int m = 0, skip = 0;
@@ -619,8 +661,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
     NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRing * sizeof(double)));
     NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
-    NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
-                          hipMemcpyHostToDevice, c->stream));
+    NH_TRY(upload_start(r0));
     std::vector<double> hz((size_t)kMaxVec * kRing);
     for (int k = 0; k < mmax; ++k) {
       NH_TRY(iterate(k, false));

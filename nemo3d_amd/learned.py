@@ -183,8 +183,13 @@ class CNN:
     def train_step(self, feats, gmap, targets, cluster, out_cluster, lr,
                    want_grad=False):
         """One Adam step (lr > 0) on the subspace loss; returns the loss
-        (and the parameter gradient when want_grad)."""
-        t = np.ascontiguousarray(targets, dtype=np.complex128)
+        (and the parameter gradient when want_grad).  targets: host array
+        or native.DeviceArray, (n_targets, N) complex128."""
+        if isinstance(targets, native.DeviceArray):
+            assert targets.dtype == np.complex128 and len(targets.shape) == 2
+            t = targets
+        else:
+            t = np.ascontiguousarray(targets, dtype=np.complex128)
         cl = np.ascontiguousarray(cluster, dtype=np.int32)
         oc = np.ascontiguousarray(out_cluster, dtype=np.int32)
         loss = ctypes.c_double()

@@ -87,6 +87,15 @@ class NHBLanczosOpts(ctypes.Structure):
     ]
 
 
+class NHRefineOpts(ctypes.Structure):
+    _fields_ = [
+        ("depth", ctypes.c_int32),
+        ("max_rounds", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("hnorm", ctypes.c_double),
+    ]
+
+
 class NHTraceminOpts(ctypes.Structure):
     _fields_ = [
         ("num_eig", ctypes.c_int32),
@@ -143,6 +152,8 @@ SIGNATURES = {
     "nh_eigensolve_lanczos": (ctypes.c_int, [P, P, P]),
     "nh_eigensolve_block_lanczos": (ctypes.c_int, [P, P, P]),
     "nh_eigensolve_tracemin": (ctypes.c_int, [P, P, P]),
+    "nh_refine_eigenpairs": (ctypes.c_int, [P, P, ctypes.c_int32, P, ctypes.c_int,
+                                            P, P, P, P]),
     "nh_vff_energy_grad": (ctypes.c_int, [P, P, P, P, P, P, P]),
     "nh_vff_relax": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_double,
                                     P, P]),

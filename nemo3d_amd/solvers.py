@@ -7,6 +7,7 @@ semantics of ``eigensolve_lanczos`` (NEMO_3D/eigen/src/lanczos.c:1823-2089);
 the iteration runs in libnemohip on the GPU.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -163,7 +164,90 @@ def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
     return out
 
 
-def eigensolve_mixed(H, block_size=8, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
+def refine_eigenpairs(H, X, eigenvalues, tol=1e-8, hnorm=40.0, depth=6,
+                      max_rounds=4):
+    """FP64 refinement of approximate eigenpairs (nh_refine_eigenpairs):
+    block Krylov [X, H X, ..., H^depth X] and Rayleigh-Ritz per round until
+    every ||H x - e x|| <= tol * hnorm.  X: (k, N) host array; returns the
+    refined values, vectors and residuals."""
+    lib = native.load()
+    Xc = np.array(X, dtype=np.complex128, order="C", copy=True)
+    k = Xc.shape[0]
+    assert Xc.shape == (k, H.n) and 1 <= k <= 16
+    ev = np.array(eigenvalues, dtype=np.float64, copy=True)
+    assert ev.shape == (k,)
+    rs = np.zeros(k)
+    o = native.NHRefineOpts()
+    o.depth, o.max_rounds = int(depth), int(max_rounds)
+    o.tolerance, o.hnorm = float(tol), float(hnorm)
+    rounds, mv = ctypes.c_int32(), ctypes.c_int32()
+    t0 = time.perf_counter()
+    st = lib.nh_refine_eigenpairs(H.ctx, ctypes.byref(o), k, native.ptr(Xc),
+                                  native.PTR_HOST, native.ptr(ev),
+                                  native.ptr(rs), ctypes.byref(rounds),
+                                  ctypes.byref(mv))
+    if st not in (native.NH_OK, 5):
+        native.check(st)
+    return {"eigenvalues": ev, "vectors": Xc, "residuals": rs,
+            "rounds": rounds.value, "matvecs": mv.value,
+            "seconds": time.perf_counter() - t0,
+            "converged": st == native.NH_OK}
+
+
+def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
+                     erange_cb=(1.4, 2.0), tol=1e-8, tol32=None, hnorm=40.0,
+                     max_iter=30000, start_iter=200, skip=50, start_vector=None,
+                     stop_rule="window", refine_depth=12, refine_rounds=4,
+                     return_vectors=False):
+    """Mixed-precision Lanczos: the Lanczos run and its eigenvector pass
+    with FP32 H·v (FP32 values and products inside each block, FP64 sums
+    over the bonds) while the recurrence, dots and the tridiagonal stay
+    FP64; then FP64 residual refinement (refine_eigenpairs) of its Ritz
+    pairs to ||H x - e x|| <= tol * hnorm.  tol32: convergence tolerance of
+    the FP32 stage (default tol).  start_vector: optional start, e.g. from
+    the learned start block."""
+    from . import native as nt
+    prec0 = H.precision
+    try:
+        H.set_precision(nt.PREC_FP32)
+        r1 = eigensolve_lanczos(
+            H, num_vb=num_vb, num_cb=num_cb, erange_vb=erange_vb,
+            erange_cb=erange_cb, max_iter=max_iter,
+            tol=tol if tol32 is None else tol32, start_iter=start_iter,
+            skip=skip, start_vector=start_vector, want_vectors=True,
+            return_vectors=True, capacity=num_vb + num_cb, stop_rule=stop_rule)
+        H.set_precision(nt.PREC_FP64)
+        n = len(r1["eigenvalues"])
+        if n == 0:
+            rf = {"eigenvalues": np.zeros(0), "vectors": np.zeros((0, H.n)),
+                  "residuals": np.zeros(0), "rounds": 0, "matvecs": 0,
+                  "seconds": 0.0, "converged": False}
+        else:
+            rf = refine_eigenpairs(H, r1["vectors"], r1["eigenvalues"], tol=tol,
+                                   hnorm=hnorm, depth=refine_depth,
+                                   max_rounds=refine_rounds)
+    finally:
+        if prec0 is not None:
+            H.set_precision(prec0)
+    order = np.argsort(rf["eigenvalues"])
+    out = {"eigenvalues": rf["eigenvalues"][order],
+           "residuals": rf["residuals"][order],
+           "iterations": r1["iterations"],
+           "matvecs": r1["matvecs"] + rf["matvecs"],
+           "seconds": r1["seconds"] + rf["seconds"],
+           "converged": bool(r1["converged"] and rf["converged"]),
+           "hv_ms": r1["hv_ms"], "hv_timed": r1["hv_timed"],
+           "stage32": {k: r1[k] for k in ("iterations", "matvecs", "seconds",
+                                          "converged", "hv_ms", "hv_timed",
+                                          "eigenvalues", "residuals")},
+           "refine": {k: rf[k] for k in ("rounds", "matvecs", "seconds",
+                                         "converged")}}
+    if return_vectors:
+        out["vectors"] = rf["vectors"][order]
+    return out
+
+
+def eigensolve_mixed_block(H, block_size=8, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                      erange_cb=(1.4, 2.0), tol=1e-8, tol32=1e-6,
                      max_iter32=2000, max_iter64=400, seed=1, start_block=None,
                      stop_rule="window", return_vectors=False, capacity=64):

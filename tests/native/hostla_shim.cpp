@@ -63,4 +63,26 @@ int hl_cholesky_upper(int p, double *G) {
   }
   return ok;
 }
+void hl_qr_r(int rows, int cols, const double *A, double *R) {
+  std::vector<cd> a((size_t)rows * cols);
+  for (size_t t = 0; t < a.size(); ++t) a[t] = cd(A[2 * t], A[2 * t + 1]);
+  const std::vector<cd> r = nh::qr_r(rows, cols, a);
+  for (size_t t = 0; t < r.size(); ++t) {
+    R[2 * t] = r[t].real();
+    R[2 * t + 1] = r[t].imag();
+  }
+}
+
+void hl_refined_ritz(int m, int q, const double *T, const double *Rw,
+                     double theta, double *y) {
+  std::vector<cd> t((size_t)m * m), rw((size_t)q * q), yy(m);
+  for (size_t k = 0; k < t.size(); ++k) t[k] = cd(T[2 * k], T[2 * k + 1]);
+  for (size_t k = 0; k < rw.size(); ++k) rw[k] = cd(Rw[2 * k], Rw[2 * k + 1]);
+  for (int k = 0; k < m; ++k) yy[k] = cd(y[2 * k], y[2 * k + 1]);
+  nh::refined_ritz(m, q, t, rw, theta, yy);
+  for (int k = 0; k < m; ++k) {
+    y[2 * k] = yy[k].real();
+    y[2 * k + 1] = yy[k].imag();
+  }
+}
 }

@@ -87,3 +87,59 @@ def test_dense_eigh_and_cholesky(lib):
     R = Gc
     assert np.abs(np.triu(R) - R).max() == 0
     assert np.abs(np.conj(R.T) @ R - G).max() < 1e-12 * np.abs(G).max()
+
+
+def test_householder_qr_r(lib):
+    lib.hl_qr_r.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(9, 5)) + 1j * rng.normal(size=(9, 5))
+    R = np.zeros((5, 5), dtype=complex)
+    lib.hl_qr_r(9, 5, P(np.ascontiguousarray(A)), P(R))
+    assert np.allclose(np.tril(R, -1), 0)
+    assert np.allclose(R.conj().T @ R, A.conj().T @ A, atol=1e-12)
+
+
+def test_refined_ritz_is_min_singular_vector(lib):
+    """Refined Ritz vector from the block Arnoldi factors (T, Rw) equals the
+    smallest right singular vector of (H - theta) B."""
+    lib.hl_refined_ritz.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    rng = np.random.default_rng(9)
+    n, q, d = 80, 2, 5
+    H = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = H + H.conj().T
+    wH, vH = np.linalg.eigh(H)
+    # start block: an eigenvector with a small error, plus a random column
+    S = rng.normal(size=(n, q)) + 1j * rng.normal(size=(n, q))
+    S[:, 0] = vH[:, 30] + 1e-6 * S[:, 0]
+    blocks = [np.linalg.qr(S)[0]]
+    m = q * (d + 1)
+    T = np.zeros((m, m), dtype=complex)
+    for i in range(d + 1):
+        W = H @ blocks[-1]
+        B = np.concatenate(blocks, axis=1)
+        for _ in range(2):
+            c = B.conj().T @ W
+            T[:c.shape[0], i * q:(i + 1) * q] += c
+            W = W - B @ c
+        if i == d:
+            break
+        Qn, R = np.linalg.qr(W)
+        T[(i + 1) * q:(i + 2) * q, i * q:(i + 1) * q] = R
+        blocks.append(Qn)
+    B = np.concatenate(blocks, axis=1)
+    # block Arnoldi relation H B = B T + W E^T
+    resid = H @ B - B @ T
+    assert np.abs(resid[:, :m - q]).max() < 1e-10
+    assert np.abs(resid[:, m - q:] - W).max() < 1e-10
+    Rw = np.linalg.cholesky(W.conj().T @ W).conj().T
+    theta = wH[30] + 1e-7
+    y = np.zeros(m, dtype=complex)
+    y[0] = 1.0
+    lib.hl_refined_ritz(m, q, P(np.ascontiguousarray(T)), P(np.ascontiguousarray(Rw)),
+                        theta, P(y))
+    _, s, vh = np.linalg.svd((H - theta * np.eye(n)) @ B)
+    v = vh[-1].conj()
+    assert abs(abs(np.vdot(v, y)) - 1.0) < 1e-10
+    assert abs(np.linalg.norm((H - theta * np.eye(n)) @ B @ y) - s[-1]) < 1e-3 * s[-1]
+    assert s[-1] < 1e-5 * s[0]
