@@ -174,7 +174,9 @@ nh_status nh_context_set(nh_context *ctx, int what, int value) {
   NH_CHECK(ctx, NH_ERR_ARG, "null context");
   switch (what) {
     case NH_SET_HV_STAGED:
-      ctx->hv_staged = value != 0;
+      NH_CHECK(value >= -1 && value <= 1, NH_ERR_ARG,
+               "NH_SET_HV_STAGED: value %d not in -1..1", value);
+      ctx->hv_staged = value;
       return NH_OK;
     default:
       NH_CHECK(false, NH_ERR_ARG, "nh_context_set: unknown setting %d", what);

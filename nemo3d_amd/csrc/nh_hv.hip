@@ -8,7 +8,7 @@ namespace {
 
 // SK-compressed layout (see nh_hv_sk.h); workgroup -> cation or anion block
 // by sk_block_of.
-template <int NSP, typename R>
+template <int NSP, typename R, bool QUAD>
 __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, int ppx,
@@ -28,10 +28,17 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
   double2 *yc = y + (int64_t)blockIdx.y * ld;
   constexpr int NO = (NB + 3) / 4;
   double own_re[NO], own_im[NO];
+  SkQuadStage<NSP, double2> *L = nullptr;
+  if constexpr (QUAD) {
+    __shared__ SkQuadStage<NSP, double2> stage;
+    L = &stage;
+  }
   if (cat)
-    sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, xc, own_re, own_im);
+    sk_bonds<NSP, R, true, double2, QUAD>(atom, slot, active, sk, nbr, bond_of,
+                                          xc, own_re, own_im, L);
   else
-    sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, xc, own_re, own_im);
+    sk_bonds<NSP, R, false, double2, QUAD>(atom, slot, active, sk, nbr, bond_of,
+                                           xc, own_re, own_im, L);
   if (!active) return;
   const double *os = onsite + (size_t)atom * kOnsiteStride;
   const double2 *xa = xc + (int64_t)atom * NB;
@@ -91,10 +98,16 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
     const dim3 g(16 * ppx, ncol);
     const double2 *xd = reinterpret_cast<const double2 *>(x);
     double2 *yd = reinterpret_cast<double2 *>(y);
+#define SKL1(NSP, R, SKP, QUAD)                                              \
+  hipLaunchKernelGGL((k_hv_sk<NSP, R, QUAD>), g, dim3(kSkThreads), 0,        \
+                     c->stream, c->n_cations, c->cations, c->n_anions,       \
+                     c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of,     \
+                     shift, xd, yd, ld)
 #define SKL(NSP, R, SKP)                                                     \
-  hipLaunchKernelGGL((k_hv_sk<NSP, R>), g, dim3(kSkThreads), 0, c->stream,   \
-                     c->n_cations, c->cations, c->n_anions, c->anions, ppx,  \
-                     c->onsite, SKP, c->nbr, c->bond_of, shift, xd, yd, ld)
+  do {                                                                       \
+    if (use_staged(c)) SKL1(NSP, R, SKP, true);                               \
+    else SKL1(NSP, R, SKP, false);                                           \
+  } while (0)
     if (c->precision == NH_PREC_FP64) {
       if (c->nb == 20) SKL(2, double, c->sk64);
       else SKL(1, double, c->sk64);
@@ -103,13 +116,14 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
       else SKL(1, float, c->sk32);
     }
 #undef SKL
+#undef SKL1
     NH_HIP(hipGetLastError());
     return NH_OK;
   }
   const dim3 grid((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock, ncol);
   const double2 *xd = reinterpret_cast<const double2 *>(x);
   double2 *yd = reinterpret_cast<double2 *>(y);
-  const bool st = c->hv_staged;
+  const bool st = use_staged(c);
 #define HVB(NSP, TB, TA, BLK)                                                  \
   do {                                                                       \
     if (st)                                                                  \

@@ -192,28 +192,99 @@ __device__ __forceinline__ void sk_load_x(const float2 *__restrict__ x,
     }
 }
 
+// Quad-cooperative gather of the neighbours' entries through LDS: the four
+// lanes of an atom load the entries of each of its four neighbours in turn,
+// lane `slot` taking the 16-byte chunks slot, slot + 4, ..., so one load
+// instruction touches one 64-byte piece per quad (16 cache lines per
+// wavefront instead of 64 when every lane reads its own neighbour); every
+// lane then reads its own neighbour's entries back from LDS.  Entries are
+// those of XT (double2: the FP64 vector; float2: its FP32 copy).
+template <int NSP, typename XT>
+struct SkQuadStage {
+  static constexpr int kChunks = NSP * 10 * (int)sizeof(XT) / 16;  // 20/10/5
+  // Lane t reads its item at stride kStride chunks: with 20 chunks the
+  // chunk index is XOR-swizzled by (t >> 2) & 3 (16 lanes then cover all 64
+  // banks; 80 KB for 256 lanes, two workgroups per CU), otherwise one chunk
+  // of padding.
+  static constexpr bool kSwizzle = kChunks % 4 == 0;
+  static constexpr int kStride = kSwizzle ? kChunks : kChunks + 1;
+  __device__ static int pos(int t, int c) {
+    return t * kStride + (kSwizzle ? (c ^ ((t >> 2) & 3)) : c);
+  }
+  uint4 buf[kSkThreads * kStride];
+};
+
+template <int NSP, typename R, typename XT>
+__device__ __forceinline__ void sk_quad_gather(int j, int slot,
+                                               const XT *__restrict__ x,
+                                               SkQuadStage<NSP, XT> &L,
+                                               R (&xr)[NSP][10],
+                                               R (&xi)[NSP][10]) {
+  using S = SkQuadStage<NSP, XT>;
+  constexpr int NB = NSP * 10;
+  constexpr int C = S::kChunks;
+  const int t = threadIdx.x, q0 = t & ~3;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int jn = __shfl(j, (t & 63 & ~3) | n, 64);
+    if (jn >= 0) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(x + (int64_t)jn * NB);
+#pragma unroll
+      for (int c0 = 0; c0 < C; c0 += 4) {
+        const int c = c0 + slot;
+        if (c < C) L.buf[S::pos(q0 + n, c)] = src[c];
+      }
+    }
+  }
+  __syncthreads();
+  if (j >= 0) {
+    if constexpr (sizeof(XT) == 16) {
+#pragma unroll
+      for (int e = 0; e < NB; ++e) {
+        const double2 v = *reinterpret_cast<const double2 *>(&L.buf[S::pos(t, e)]);
+        xr[e / 10][e % 10] = R(v.x);
+        xi[e / 10][e % 10] = R(v.y);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < NB; e += 2) {
+        const float4 v = *reinterpret_cast<const float4 *>(&L.buf[S::pos(t, e / 2)]);
+        xr[e / 10][e % 10] = R(v.x);
+        xi[e / 10][e % 10] = R(v.y);
+        xr[(e + 1) / 10][(e + 1) % 10] = R(v.z);
+        xi[(e + 1) / 10][(e + 1) % 10] = R(v.w);
+      }
+    }
+  }
+}
+
 // The 4 bonds of `atom` (lane = slot), summed over the quad in FP64 (fixed
 // order); lane `slot` keeps rows r = 4 i + slot in own_re[i] / own_im[i].
 // XT: double2 (the FP64 vector) or float2 (its FP32 copy, R = float).
-template <int NSP, typename R, bool CATION, typename XT>
+// QUAD: gather through LDS (sk_quad_gather; every thread of the workgroup
+// must call, there is a barrier) in the caller's LDS area L, or directly
+// from the caches.
+template <int NSP, typename R, bool CATION, typename XT, bool QUAD>
 __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
                                          const R *__restrict__ sk,
                                          const int32_t *__restrict__ nbr,
                                          const int32_t *__restrict__ bond_of,
                                          const XT *__restrict__ xc,
                                          double (&own_re)[(NSP * 10 + 3) / 4],
-                                         double (&own_im)[(NSP * 10 + 3) / 4]) {
+                                         double (&own_im)[(NSP * 10 + 3) / 4],
+                                         SkQuadStage<NSP, XT> *L = nullptr) {
   R br[NSP][10], bi[NSP][10];
 #pragma unroll
   for (int s = 0; s < NSP; ++s)
 #pragma unroll
     for (int c = 0; c < 10; ++c) br[s][c] = bi[s][c] = R(0);
   const int j = active ? nbr[atom * 4 + slot] : -1;
+  R xr[NSP][10], xi[NSP][10];
+  if constexpr (QUAD) sk_quad_gather<NSP, R, XT>(j, slot, xc, *L, xr, xi);
   if (j >= 0) {
     R rv[24];
     sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
-    R xr[NSP][10], xi[NSP][10];
-    sk_load_x<NSP, R>(xc, j, xr, xi);
+    if constexpr (!QUAD) sk_load_x<NSP, R>(xc, j, xr, xi);
     BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3, xr,
                                                 xi, br, bi);
   }

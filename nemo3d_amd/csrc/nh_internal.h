@@ -80,7 +80,7 @@ struct nh_context {
   // stored-block H·v: neighbour x staged in LDS (measured on the 786,520-
   // atom dot: faster for the plain product, slower inside the fused Lanczos
   // kernel, so off by default)
-  bool hv_staged = false;
+  int hv_staged = -1;  // NH_SET_HV_STAGED (-1: by layout, see use_staged)
 
   // assembled Hamiltonian
   int layout = -1;
@@ -101,6 +101,16 @@ struct nh_context {
   nh_cplx *ybuf = nullptr;
   int64_t buf_cap = 0;          // complex elements in xbuf/ybuf
 };
+
+namespace nh {
+// LDS staging of the neighbours' entries in H·v: measured faster for the SK
+// layout (quad gather), slower for the stored-block layout inside the
+// fused Lanczos step; NH_SET_HV_STAGED overrides.
+inline bool use_staged(const nh_context *c) {
+  return c->hv_staged < 0 ? c->layout == NH_LAYOUT_SK : c->hv_staged != 0;
+}
+}  // namespace nh
+
 
 namespace nh {
 // Sampled kernel timing with HIP events on the launching stream: begin()/

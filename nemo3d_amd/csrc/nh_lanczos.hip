@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "nh_hv_row.h"
@@ -89,7 +90,24 @@ __global__ __launch_bounds__(kThreads) void k_lz_hv(
 
 // SK-layout variant of k_lz_hv (mapping of nh_hv_sk.h: 4 lanes per atom,
 // cation and anion workgroups).
-template <int NSP, typename R>
+// Workgroup sum for the fused SK kernel: with the LDS gather the staging
+// area is reused (the kernel's LDS stays at 80 KB for two groups per CU).
+template <bool QUAD>
+struct LzBlockSum {
+  template <typename S>
+  __device__ __forceinline__ static void run(double &a, double &b, S *) {
+    block_sum2(a, b);
+  }
+};
+template <>
+struct LzBlockSum<true> {
+  template <typename S>
+  __device__ __forceinline__ static void run(double &a, double &b, S *L) {
+    block_sum2_lds(a, b, reinterpret_cast<double *>(L->buf));
+  }
+};
+
+template <int NSP, typename R, bool QUAD>
 __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, int ppx,
@@ -111,17 +129,21 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
   double own_re[NO], own_im[NO];
   // FP32 products gather from the FP32 copy of r (the same rounding as
   // converting the FP64 entries on load)
-  if constexpr (sizeof(R) == 4) {
-    if (cat)
-      sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, Rv32, own_re, own_im);
-    else
-      sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, Rv32, own_re, own_im);
-  } else {
-    if (cat)
-      sk_bonds<NSP, R, true>(atom, slot, active, sk, nbr, bond_of, Rv, own_re, own_im);
-    else
-      sk_bonds<NSP, R, false>(atom, slot, active, sk, nbr, bond_of, Rv, own_re, own_im);
+  using XT = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+  const XT *xg;
+  if constexpr (sizeof(R) == 4) xg = Rv32;
+  else xg = Rv;
+  SkQuadStage<NSP, XT> *L = nullptr;
+  if constexpr (QUAD) {
+    __shared__ SkQuadStage<NSP, XT> stage;
+    L = &stage;
   }
+  if (cat)
+    sk_bonds<NSP, R, true, XT, QUAD>(atom, slot, active, sk, nbr, bond_of, xg,
+                                     own_re, own_im, L);
+  else
+    sk_bonds<NSP, R, false, XT, QUAD>(atom, slot, active, sk, nbr, bond_of, xg,
+                                      own_re, own_im, L);
   double local = 0.0;
   if (active) {
     const double *os = onsite + (size_t)atom * kOnsiteStride;
@@ -142,7 +164,7 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
   }
   if (part) {
     double z = 0.0;
-    block_sum2(local, z);
+    LzBlockSum<QUAD>::run(local, z, L);  // QUAD: reuses the staging LDS
     if (threadIdx.x == 0) part[blockIdx.x] = local;
   }
 }
@@ -410,7 +432,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
   }
 #define LZ_HV(NSP, TB, BLK)                                                    \
   do {                                                                         \
-    if (c->hv_staged)                                                          \
+    if (use_staged(c))                                                          \
       hipLaunchKernelGGL((k_lz_hv<NSP, TB, TB, true>), dim3(gridA),            \
                          dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
                          BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
@@ -419,11 +441,16 @@ nh_status LanczosRun::iterate(int k, bool red) {
                          dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
                          BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
   } while (0)
-#define LZ_SK(NSP, R, SKP)                                                      \
-  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R>), dim3(gridA), dim3(kSkThreads), 0,    \
-                     c->stream, c->n_cations, c->cations, c->n_anions,          \
+#define LZ_SK1(NSP, R, SKP, QUAD)                                               \
+  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD>), dim3(gridA), dim3(kSkThreads),  \
+                     0, c->stream, c->n_cations, c->cations, c->n_anions,       \
                      c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of, R_,    \
                      R32_, Q, Y_, bprev, part)
+#define LZ_SK(NSP, R, SKP)                                                      \
+  do {                                                                          \
+    if (use_staged(c)) LZ_SK1(NSP, R, SKP, true);                                \
+    else LZ_SK1(NSP, R, SKP, false);                                            \
+  } while (0)
   if (c->layout == NH_LAYOUT_SK) {
     if (c->precision == NH_PREC_FP64) {
       if (c->nb == 20) LZ_SK(2, double, c->sk64);
@@ -441,6 +468,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
   }
 #undef LZ_HV
 #undef LZ_SK
+#undef LZ_SK1
   if (timed) {
     NH_HIP(hipEventRecord(evp.second, c->stream));
     ev_.push_back(evp);

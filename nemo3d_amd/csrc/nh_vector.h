@@ -38,6 +38,34 @@ __device__ __forceinline__ void block_sum2(double &a, double &b) {
   }
 }
 
+// block_sum2 over kRedThreads threads with caller-provided LDS scratch
+// (2 * kRedThreads / 64 doubles, free for reuse: a barrier comes first).
+__device__ __forceinline__ void block_sum2_lds(double &a, double &b,
+                                               double *scratch) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    scratch[w] = a;
+    scratch[kRedThreads / 64 + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0.0, tb = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRedThreads / 64; ++k) {
+      ta += scratch[k];
+      tb += scratch[kRedThreads / 64 + k];
+    }
+    a = ta;
+    b = tb;
+  }
+}
+
 // Final sum of `nparts` (re, im) partials into out[0..1].
 static __global__ __launch_bounds__(kRedThreads) void k_sum_partials(
     int nparts, const double *__restrict__ part, double *__restrict__ out) {

@@ -187,3 +187,46 @@ def test_hv_staged_and_cached_variants_agree(case, precision):
         yr = ref.hv(X[k], 0.2)
         err = np.linalg.norm(Ys[k] - yr) / np.linalg.norm(yr)
         assert err <= (1e-12 if precision == "fp64" else 2e-6), err
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_hv_sk_quad_gather_matches_oracle(case, precision):
+    """SK layout with the quad-cooperative LDS gather of the neighbours'
+    entries: same products as the direct gather (same arithmetic order),
+    oracle to the FP64 / FP32 bar, single and 3-column products."""
+    M = make(case)
+    print("case", case, precision, flush=True)
+    ref = oracle.assemble(M.arrays())
+    H = Hamiltonian(M, layout=native.LAYOUT_SK,
+                    precision=native.PREC_FP64 if precision == "fp64" else native.PREC_FP32)
+    X = _models.random_vectors(H.n, 3, seed=17)
+    H.set(native.SET_HV_STAGED, 1)
+    Ys = H.hV(X, 0.3)
+    y1 = H.hv(X[1], 0.3)
+    H.set(native.SET_HV_STAGED, 0)
+    Yc = H.hV(X, 0.3)
+    assert np.array_equal(Ys, Yc) and np.array_equal(y1, Yc[1])
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    for k in range(3):
+        yr = ref.hv(X[k], 0.3)
+        assert np.linalg.norm(Ys[k] - yr) <= tol * np.linalg.norm(yr)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lanczos_sk_quad_gather_same_run(precision):
+    """The fused Lanczos kernels with the quad LDS gather give the same
+    tridiagonal as the direct gather (bit for bit) and the dense eigenvalues
+    bracketing the gap."""
+    from nemo3d_amd.solvers import eigensolve_lanczos
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=23)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK,
+                    precision=native.PREC_FP64 if precision == "fp64" else native.PREC_FP32)
+    opts = dict(num_vb=0, num_cb=1, erange_cb=(1e3, 1e3 + 1), max_iter=400,
+                start_iter=10 ** 6, want_vectors=False, return_tridiag=True)
+    H.set(native.SET_HV_STAGED, 0)
+    r0 = eigensolve_lanczos(H, **opts)
+    H.set(native.SET_HV_STAGED, 1)
+    r1 = eigensolve_lanczos(H, **opts)
+    H.set(native.SET_HV_STAGED, 0)
+    assert np.array_equal(r0["alpha"], r1["alpha"]) and np.array_equal(r0["beta"], r1["beta"])
