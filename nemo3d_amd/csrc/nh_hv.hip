@@ -39,6 +39,27 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
   else
     sk_bonds<NSP, R, false, double2, QUAD>(atom, slot, active, sk, nbr, bond_of,
                                            xc, own_re, own_im, L);
+  if constexpr (QUAD) {
+    auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
+    const int la = threadIdx.x / 4;
+    double2 own_x[NO];
+    __syncthreads();  // every lane has read its staged neighbour entries
+    if (active)
+      sk_stage_own<NSP>(la, slot, xc + (int64_t)atom * NB,
+                        onsite + (size_t)atom * kOnsiteStride, S, own_x);
+    __syncthreads();
+    if (!active) return;
+#pragma unroll
+    for (int i = 0; i < NO; ++i) {
+      const int r = 4 * i + slot;
+      if (r >= NB) break;
+      double re = own_re[i], im = own_im[i];
+      onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la],
+                      r / 10, r % 10, shift, re, im);
+      yc[(int64_t)atom * NB + r] = make_double2(re, im);
+    }
+    return;
+  }
   if (!active) return;
   const double *os = onsite + (size_t)atom * kOnsiteStride;
   const double2 *xa = xc + (int64_t)atom * NB;

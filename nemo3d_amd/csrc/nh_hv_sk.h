@@ -258,6 +258,38 @@ __device__ __forceinline__ void sk_quad_gather(int j, int slot,
   }
 }
 
+// Epilogue staging (after the quad gather, in the same LDS): every atom's
+// own entries and onsite record, loaded by its quad with lane `slot` taking
+// rows slot, slot + 4, ... (64 contiguous bytes per quad and load), so that
+// the onsite couplings read their partner entries from LDS instead of
+// issuing scattered loads per row.
+template <int NSP>
+struct SkOwnStage {
+  double2 x[kSkAtomsPerBlock][NSP * 10];
+  double2 os[kSkAtomsPerBlock][kOnsiteStride / 2];
+};
+
+template <int NSP>
+__device__ __forceinline__ void sk_stage_own(int la, int slot,
+                                             const double2 *__restrict__ xa,
+                                             const double *__restrict__ os,
+                                             SkOwnStage<NSP> &S,
+                                             double2 (&own_x)[(NSP * 10 + 3) / 4]) {
+  constexpr int NB = NSP * 10;
+#pragma unroll
+  for (int i = 0; i < (NB + 3) / 4; ++i) {
+    const int r = 4 * i + slot;
+    if (r < NB) {
+      own_x[i] = xa[r];
+      S.x[la][r] = own_x[i];
+    }
+  }
+  const double2 *o2 = reinterpret_cast<const double2 *>(os);  // 144-byte records
+#pragma unroll
+  for (int c0 = 0; c0 < kOnsiteStride / 2; c0 += 4)
+    if (c0 + slot < kOnsiteStride / 2) S.os[la][c0 + slot] = o2[c0 + slot];
+}
+
 // The 4 bonds of `atom` (lane = slot), summed over the quad in FP64 (fixed
 // order); lane `slot` keeps rows r = 4 i + slot in own_re[i] / own_im[i].
 // XT: double2 (the FP64 vector) or float2 (its FP32 copy, R = float).

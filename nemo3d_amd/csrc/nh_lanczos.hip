@@ -145,7 +145,33 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
     sk_bonds<NSP, R, false, XT, QUAD>(atom, slot, active, sk, nbr, bond_of, xg,
                                       own_re, own_im, L);
   double local = 0.0;
-  if (active) {
+  if constexpr (QUAD) {
+    static_assert(sizeof(SkOwnStage<NSP>) <= sizeof(SkQuadStage<NSP, XT>),
+                  "epilogue staging must fit the gather staging");
+    auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
+    const int la = threadIdx.x / 4;
+    double2 own_x[NO];
+    __syncthreads();  // every lane has read its staged neighbour entries
+    if (active)
+      sk_stage_own<NSP>(la, slot, Rv + (int64_t)atom * NB,
+                        onsite + (size_t)atom * kOnsiteStride, S, own_x);
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < NO; ++i) {
+        const int r = 4 * i + slot;
+        if (r >= NB) break;
+        double re = own_re[i], im = own_im[i];
+        onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la],
+                        r / 10, r % 10, 0.0, re, im);
+        const double2 q = make_double2(own_x[i].x * inv, own_x[i].y * inv);
+        const double2 y = make_double2(re * inv, im * inv);
+        Q[(int64_t)atom * NB + r] = q;
+        Y[(int64_t)atom * NB + r] = y;
+        local += q.x * y.x + q.y * y.y;
+      }
+    }
+  } else if (active) {
     const double *os = onsite + (size_t)atom * kOnsiteStride;
     const double2 *xa = Rv + (int64_t)atom * NB;
 #pragma unroll
