@@ -55,16 +55,18 @@ def build_dot():
     return M, info
 
 
-def sk_lanczos_bytes(M, vec_bytes=16, rec_bytes=8):
-    """Algorithmic bytes of one fused Lanczos H·v launch in the SK layout:
-    onsite record (18 doubles) per atom, one 24-value record per bond
-    (each bond once; rec_bytes per value), neighbour and bond indices
-    (4 + 4 int32 per atom), the atom lists, x read once, q and y written
-    once."""
+def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False):
+    """Algorithmic bytes of one fused Lanczos H·v launch of the eigenvalue
+    pass in the SK layout: onsite record (18 doubles) per atom, one 24-value
+    record per bond (each bond once; rec_bytes per value), neighbour and
+    bond indices (4 + 4 int32 per atom), the atom lists, x read once (FP64)
+    and y written once (q is stored only in the eigenvector pass); with
+    FP32 products the neighbours' entries come from the FP32 copy of x
+    (8 bytes per entry, read once)."""
     A = M.n_atoms
     nb = int((M.structure.nbr >= 0).sum()) // 2
     mat = A * 18 * 8 + nb * 24 * rec_bytes + A * 4 * 4 * 2 + A * 4
-    return mat + 3 * M.n * vec_bytes
+    return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
 def cpu_baseline(M, matvecs, sample_iters=2):
@@ -211,7 +213,7 @@ def main():
             cfg["speedup_vs_plain"] = cfg["plain_fp64_s"] / cfg["mixed_s"]
 
     if mixed is not None:
-        nbytes = sk_lanczos_bytes(M, rec_bytes=4)
+        nbytes = sk_lanczos_bytes(M, rec_bytes=4, fp32_copy=True)
         kms = mixed["stage32"]["hv_ms"]
         kname = "k_lz_hv_sk<2,float> (fused Lanczos H*v, SK FP32 products)"
         tr_key = "k_lz_hv_sk_fp32"

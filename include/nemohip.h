@@ -77,8 +77,8 @@ nh_status nh_context_create(const nh_model *model, int device,
 void nh_context_destroy(nh_context *ctx);
 int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
 
-/* Kernel variants (chosen by measured throughput; both give the same
- * products, bit for bit).  NH_SET_HV_STAGED: 1 stages the neighbours'
+/* Kernel variants (chosen by measured throughput; the same products, bit
+ * for bit in FP64 and to rounding in FP32).  NH_SET_HV_STAGED: 1 stages the neighbours'
  * vector entries in LDS (stored-block layout: per workgroup; SK layout: the
  * four lanes of an atom load each neighbour's entries together), 0 reads
  * them from the caches by every lane, -1 (default) picks by layout (SK:

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void k_lz_hv(
     for (int s = 0; s < NSP; ++s) {
       const double2 q = make_double2(xi[s].x * inv, xi[s].y * inv);
       const double2 y = make_double2(acc_re[s] * inv, acc_im[s] * inv);
-      Q[base + s * 10 + o] = q;
+      if (Q) Q[base + s * 10 + o] = q;
       Y[base + s * 10 + o] = y;
       local += q.x * y.x + q.y * y.y;
     }
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
                         r / 10, r % 10, 0.0, re, im);
         const double2 q = make_double2(own_x[i].x * inv, own_x[i].y * inv);
         const double2 y = make_double2(re * inv, im * inv);
-        Q[(int64_t)atom * NB + r] = q;
+        if (Q) Q[(int64_t)atom * NB + r] = q;
         Y[(int64_t)atom * NB + r] = y;
         local += q.x * y.x + q.y * y.y;
       }
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
       const double2 xr = xa[r];
       const double2 q = make_double2(xr.x * inv, xr.y * inv);
       const double2 y = make_double2(re * inv, im * inv);
-      Q[(int64_t)atom * NB + r] = q;
+      if (Q) Q[(int64_t)atom * NB + r] = q;
       Y[(int64_t)atom * NB + r] = y;
       local += q.x * y.x + q.y * y.y;
     }
@@ -195,22 +195,30 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
   }
 }
 
-// y <- y - alpha q - beta_prev q_prev, partial sums of |y|^2.
+// y <- y - alpha q - beta_prev q_prev with q = r * (1 / beta_prev) and
+// q_prev = r_prev * (1 / beta_prev2) formed on the fly (the same products
+// as the fused H·v kernel's q, so the trajectory does not depend on whether
+// q was stored), partial sums of |y|^2.  Rp may be null (first step), and
+// beta_prev / beta_prev2 null mean 1 (the start vector is normalised).
 // Y32 (optional): FP32 copy of the new r for the next FP32 product.
 __global__ __launch_bounds__(kRedThreads) void k_lz_update(
-    int64_t n, double2 *__restrict__ Y, const double2 *__restrict__ Q,
-    const double2 *__restrict__ Qp, const double *__restrict__ alpha,
-    const double *__restrict__ beta_prev, double *__restrict__ part,
-    float2 *__restrict__ Y32) {
+    int64_t n, double2 *__restrict__ Y, const double2 *__restrict__ Rk,
+    const double2 *__restrict__ Rp, const double *__restrict__ alpha,
+    const double *__restrict__ beta_prev, const double *__restrict__ beta_prev2,
+    double *__restrict__ part, float2 *__restrict__ Y32) {
   const double al = alpha[0];
   const double be = beta_prev ? beta_prev[0] : 0.0;
+  const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
+  const double invp = beta_prev2 ? 1.0 / beta_prev2[0] : 1.0;
   double s = 0.0, z = 0.0;
   for (int64_t k = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * kRedThreads) {
     double2 y = Y[k];
-    const double2 q = Q[k];
-    if (Qp) {
-      const double2 p = Qp[k];
+    const double2 r = Rk[k];
+    const double2 q = make_double2(r.x * inv, r.y * inv);
+    if (Rp) {
+      const double2 rp = Rp[k];
+      const double2 p = make_double2(rp.x * invp, rp.y * invp);
       y.x -= al * q.x + be * p.x;
       y.y -= al * q.y + be * p.y;
     } else {
@@ -378,7 +386,7 @@ class LanczosRun {
   nh_context *c_;
   std::vector<void *> allocs_;
   int64_t n_ = 0;
-  double2 *R_ = nullptr, *Y_ = nullptr, *ring_ = nullptr;
+  double2 *R_ = nullptr, *Y_ = nullptr, *Rm1_ = nullptr, *ring_ = nullptr;
   // FP32 copies of r for the FP32 SK products (null otherwise)
   float2 *R32_ = nullptr, *Y32_ = nullptr;
   nh_status upload_start(const std::vector<nh_cplx> &r0);
@@ -445,9 +453,11 @@ nh_status LanczosRun::iterate(int k, bool red) {
   const int gridA = c->layout == NH_LAYOUT_SK
                         ? 16 * ppx
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
-  double2 *Q = ring_ + (int64_t)(k % kRing) * n_;
-  double2 *Qp = k > 0 ? ring_ + (int64_t)((k - 1) % kRing) * n_ : nullptr;
+  // q_k is stored (ring slot) only in the eigenvector pass, which
+  // accumulates from it; the update forms q_k, q_{k-1} from r_k, r_{k-1}
+  double2 *Q = red ? nullptr : ring_ + (int64_t)(k % kRing) * n_;
   const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
+  const double *bprev2 = k > 1 ? beta_ + (k - 2) : nullptr;
   double *part = red ? part_ : nullptr;
   const bool timed = red && (k % kTimeEvery) == 0;
   std::pair<hipEvent_t, hipEvent_t> evp;
@@ -504,12 +514,17 @@ nh_status LanczosRun::iterate(int k, bool red) {
                        gridA, 1, part_, alpha_ + k, 0);
   const int nb = reduce_blocks(n_);
   hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
-                     n_, Y_, Q, Qp, alpha_ + k, bprev, part, Y32_);
+                     n_, Y_, R_, k > 0 ? Rm1_ : nullptr, alpha_ + k, bprev,
+                     bprev2, part, Y32_);
   if (red)
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        nb, 2, part_, beta_ + k, 1);
   NH_HIP(hipGetLastError());
-  std::swap(R_, Y_);  // new r is in Y; old r buffer becomes next y
+  // r_{k+1} is in Y: rotate (r_{k-1}, r_k, y) -> (r_k, r_{k+1}, free)
+  double2 *free_buf = Rm1_;
+  Rm1_ = R_;
+  R_ = Y_;
+  Y_ = free_buf;
   std::swap(R32_, Y32_);
   return NH_OK;
 }
@@ -599,6 +614,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   n_ = c->n;
   NH_TRY(alloc((void **)&R_, n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&Y_, n_ * sizeof(double2)));
+  NH_TRY(alloc((void **)&Rm1_, n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&ring_, (size_t)kRing * n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&alpha_, (size_t)(o->max_iter + 1) * sizeof(double)));
   NH_TRY(alloc((void **)&beta_, (size_t)(o->max_iter + 1) * sizeof(double)));

@@ -54,9 +54,12 @@ def find_gap(H, iters=2000, lo=-1.0, hi=3.0, tol=1e-6):
     a, b = r["alpha"], r["beta"]
     w, z = eigh_tridiagonal(a, b[:len(a) - 1], select="v",
                             select_range=(lo, hi))
-    w = w[np.abs(z[-1]) < tol]
+    conv = w[np.abs(z[-1]) < tol]
+    # interior Ritz values of a large structure converge late: with too few
+    # converged ones, use all of them (Ritz values rarely sit in the gap)
+    w = conv if len(conv) >= 8 else w
     if len(w) < 2:
-        raise RuntimeError("find_gap: too few converged Ritz values")
+        raise RuntimeError("find_gap: too few Ritz values in the range")
     k = int(np.argmax(np.diff(w)))
     return float(w[k]), float(w[k + 1])
 
@@ -167,6 +170,10 @@ def main():
     ap.add_argument("--box", type=float, nargs=3, default=(24.0, 24.0, 16.0))
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--vb-window", type=float, nargs=2, default=(0.3, 0.95),
+                    help="valence window (eV); its top lies in the gap")
+    ap.add_argument("--cb-window", type=float, nargs=2, default=(1.0, 1.8),
+                    help="conduction window (eV); its bottom lies in the gap")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
     samples, meta = [], []
@@ -176,10 +183,18 @@ def main():
         M = dot_model(tuple(a.box), p["base_nm"], p["height_nm"], p["x_in"], p["seed"])
         info = vff.relax(M)
         H = Hamiltonian(M)
-        r = plain_levels(H)
-        if not r["converged"] or len(r["eigenvalues"]) < 8:
+        r = plain_levels(H, erange_vb=tuple(a.vb_window),
+                         erange_cb=tuple(a.cb_window))
+        # distinct levels only (a converged copy of a level counts once)
+        ev, keep = r["eigenvalues"], []
+        for i in np.argsort(ev):
+            if all(abs(ev[i] - ev[k]) >= 1e-7 for k in keep):
+                keep.append(i)
+        if not r["converged"] or len(keep) < 6:
             print("dot %d: plain solver did not converge, skipped" % k, flush=True)
             continue
+        r["eigenvalues"] = r["eigenvalues"][keep]
+        r["vectors"] = r["vectors"][keep]
         samples.append(sample(M, r["eigenvalues"], r["vectors"], owner=H))
         meta.append(dict(p, atoms=M.n_atoms, vff_iter=info["iterations"],
                          levels=[float(e) for e in r["eigenvalues"]],
