@@ -211,18 +211,7 @@ struct SkQuadStage {
   __device__ static int pos(int t, int c) {
     return t * kStride + (kSwizzle ? (c ^ ((t >> 2) & 3)) : c);
   }
-  // bond records (24 values of R) staged the same way afterwards
-  template <typename R>
-  struct Rec {
-    static constexpr int kChunks = 24 * (int)sizeof(R) / 16;  // 12 / 6
-    static constexpr bool kSwizzle = kChunks % 4 == 0;
-    static constexpr int kStride = kSwizzle ? kChunks : kChunks + 1;
-    __device__ static int pos(int t, int c) {
-      return t * kStride + (kSwizzle ? (c ^ ((t >> 2) & 3)) : c);
-    }
-  };
-  static constexpr int kCap = kStride > 12 ? kStride : 12;  // >= any Rec stride
-  uint4 buf[kSkThreads * kCap];
+  uint4 buf[kSkThreads * kStride];
 };
 
 template <int NSP, typename R, typename XT>
@@ -264,56 +253,6 @@ __device__ __forceinline__ void sk_quad_gather(int j, int slot,
         xi[e / 10][e % 10] = R(v.y);
         xr[(e + 1) / 10][(e + 1) % 10] = R(v.z);
         xi[(e + 1) / 10][(e + 1) % 10] = R(v.w);
-      }
-    }
-  }
-}
-
-// Quad-cooperative load of the four bond records of each atom through
-// LDS, after the neighbours' entries have been read back (barrier first):
-// one 64-byte piece of one record per quad and load instead of 64 separate
-// records per wavefront.  bond < 0: no record (the lane's values are 0).
-template <int NSP, typename R, typename XT>
-__device__ __forceinline__ void sk_quad_records(int bond, int slot,
-                                                const R *__restrict__ sk,
-                                                SkQuadStage<NSP, XT> &L,
-                                                R (&rv)[24]) {
-  using S = typename SkQuadStage<NSP, XT>::template Rec<R>;
-  constexpr int C = S::kChunks;
-  const int t = threadIdx.x, q0 = t & ~3;
-  __syncthreads();  // every lane has read its staged neighbour entries
-#pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    const int bn = __shfl(bond, (t & 63 & ~3) | n, 64);
-    if (bn >= 0) {
-      const uint4 *src =
-          reinterpret_cast<const uint4 *>(sk + (size_t)bn * kSkStride);
-#pragma unroll
-      for (int c0 = 0; c0 < C; c0 += 4) {
-        const int c = c0 + slot;
-        if (c < C) L.buf[S::pos(q0 + n, c)] = src[c];
-      }
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 24; ++k) rv[k] = R(0);
-  if (bond >= 0) {
-    if constexpr (sizeof(R) == 8) {
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const double2 v = *reinterpret_cast<const double2 *>(&L.buf[S::pos(t, c)]);
-        rv[2 * c] = v.x;
-        rv[2 * c + 1] = v.y;
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const float4 v = *reinterpret_cast<const float4 *>(&L.buf[S::pos(t, c)]);
-        rv[4 * c] = v.x;
-        rv[4 * c + 1] = v.y;
-        rv[4 * c + 2] = v.z;
-        rv[4 * c + 3] = v.w;
       }
     }
   }
@@ -375,18 +314,12 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
   R xr[NSP][10], xi[NSP][10];
   if constexpr (QUAD) {
     sk_quad_gather<NSP, R, XT>(j, slot, xc, *L, xr, xi);
-    R rv[24];
-    if constexpr (sizeof(R) == 4) {
-      // FP32 records through LDS too (the FP64 kernel has no registers to
-      // spare for it: 256 VGPRs with the direct 16-byte record loads)
-      sk_quad_records<NSP, R, XT>(j >= 0 ? bond_of[atom * 4 + slot] : -1,
-                                  slot, sk, *L, rv);
-    } else if (j >= 0) {
+    if (j >= 0) {
+      R rv[24];
       sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
-    }
-    if (j >= 0)
       BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3,
                                                   xr, xi, br, bi);
+    }
   } else if (j >= 0) {
     R rv[24];
     sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
