@@ -170,7 +170,8 @@ typedef struct nh_eig_result {
   int32_t matvecs;       /* H·v products, all passes */
   double *eigenvalues;   /* caller array, capacity `capacity` */
   double *residuals;     /* caller array: ||H psi - E psi|| / ||psi|| */
-  nh_cplx *eigenvectors; /* caller host array capacity*N, or NULL */
+  nh_cplx *eigenvectors; /* caller array capacity*N (host or device
+                            memory), or NULL */
   int32_t capacity;
   double seconds;        /* wall time of the solve */
   double hv_ms;          /* mean duration of the fused H·v kernel (HIP

@@ -1,5 +1,7 @@
 // C-ABI entry points of libnemohip (declared in include/nemohip.h).
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -69,6 +71,49 @@ static void destroy(nh_context *c) {
   delete c;
 }
 
+// Orders an atom list by 8x8x8-cell bricks (bricks in z, y, x order, cells
+// and atoms in their original order inside a brick).  The SK kernels give
+// consecutive workgroups consecutive list entries and each XCD a contiguous
+// part of the list, so a brick's atoms and most of their neighbours are in
+// flight together and the neighbours' vector entries are reused from the
+// XCD's L2 instead of being fetched again when the next plane of cells is
+// processed.  Ideal positions (units a/4) come from the ideal bond vectors
+// by a walk over the bond graph.
+static void brick_order(const nh_model *m, std::vector<int32_t> &list) {
+  const int A = m->n_atoms;
+  std::vector<int32_t> pos((size_t)A * 3, 0);
+  std::vector<char> seen(A, 0);
+  std::vector<int32_t> stack;
+  for (int s0 = 0; s0 < A; ++s0) {
+    if (seen[s0]) continue;
+    seen[s0] = 1;
+    stack.push_back(s0);
+    while (!stack.empty()) {
+      const int i = stack.back();
+      stack.pop_back();
+      for (int n = 0; n < 4; ++n) {
+        const int j = m->nbr[i * 4 + n];
+        if (j < 0 || seen[j]) continue;
+        seen[j] = 1;
+        for (int d = 0; d < 3; ++d)
+          pos[(size_t)j * 3 + d] = pos[(size_t)i * 3 + d] + m->bond_q[(i * 4 + n) * 3 + d];
+        stack.push_back(j);
+      }
+    }
+  }
+  int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
+  for (int i = 0; i < A; ++i)
+    for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], pos[(size_t)i * 3 + d]);
+  constexpr int kBrick = 32;  // 8 cells of 4 units (a/4)
+  auto key = [&](int32_t i) {
+    int64_t b[3];
+    for (int d = 0; d < 3; ++d) b[d] = (pos[(size_t)i * 3 + d] - lo[d]) / kBrick;
+    return (b[2] << 42) | (b[1] << 21) | b[0];
+  };
+  std::stable_sort(list.begin(), list.end(),
+                   [&](int32_t a, int32_t b) { return key(a) < key(b); });
+}
+
 static nh_status create(const nh_model *m, int device, nh_context **out) {
   NH_CHECK(m && out, NH_ERR_ARG, "null model or output pointer");
   NH_CHECK(m->nb == 10 || m->nb == 20, NH_ERR_ARG, "nb must be 10 or 20");
@@ -103,6 +148,8 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
   }
   for (int i = 0; i < A; ++i)
     (m->is_cation[i] ? cations : anions).push_back(i);
+  brick_order(m, cations);
+  brick_order(m, anions);
   for (size_t ci = 0; ci < cations.size(); ++ci) {
     const int i = cations[ci];
     for (int n = 0; n < 4; ++n) {

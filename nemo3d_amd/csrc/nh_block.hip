@@ -454,7 +454,7 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
       if (res->residuals) res->residuals[e] = std::sqrt(r2);
       if (res->eigenvectors)
         NH_HIP(hipMemcpy(res->eigenvectors + (size_t)e * n_, v,
-                         n_ * sizeof(nh_cplx), hipMemcpyDeviceToHost));
+                         n_ * sizeof(nh_cplx), hipMemcpyDefault));  // host or device
     }
   }
   NH_HIP(hipStreamSynchronize(c->stream));

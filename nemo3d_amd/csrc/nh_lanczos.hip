@@ -775,7 +775,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
       if (res->residuals) res->residuals[e] = std::sqrt(r2);
       if (res->eigenvectors)
         NH_HIP(hipMemcpy(res->eigenvectors + (int64_t)e * n_, v,
-                         n_ * sizeof(nh_cplx), hipMemcpyDeviceToHost));
+                         n_ * sizeof(nh_cplx), hipMemcpyDefault));  // host or device
     }
   }
   NH_HIP(hipStreamSynchronize(c->stream));

@@ -323,7 +323,7 @@ nh_status Tracemin::run(const nh_tracemin_opts *o, nh_eig_result *res) {
     if (res->residuals) res->residuals[e] = resid[j];
     if (res->eigenvectors)
       NH_HIP(hipMemcpy(res->eigenvectors + (size_t)e * n_, X + (size_t)j * n_,
-                       n_ * sizeof(nh_cplx), hipMemcpyDeviceToHost));
+                       n_ * sizeof(nh_cplx), hipMemcpyDefault));  // host or device
   }
   res->matvecs = matvecs_;
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

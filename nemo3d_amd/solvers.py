@@ -22,7 +22,8 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                        start_iter=200, skip=50, start="CyclicRandom",
                        cyclic=DEFAULT_CYCLIC, start_vector=None,
                        want_vectors=True, capacity=64, return_vectors=False,
-                       return_tridiag=False, stop_rule="window"):
+                       return_tridiag=False, stop_rule="window",
+                       vectors_out=None):
     """stop_rule "window": stop once num_vb / num_cb converged values lie in
     the windows (the reference's rule, lanczos.c:1231-1232); "nearest":
     stop once the num_vb / num_cb Ritz values nearest the gap are all
@@ -49,6 +50,9 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     ev = np.zeros(capacity)
     rs = np.full(capacity, -1.0)
     vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
+    if vectors_out is not None:  # device array (capacity, N): vectors stay on the GPU
+        assert vectors_out.shape == (capacity, H.n) and vec is None
+        vec = vectors_out
     r = native.NHEigResult()
     r.eigenvalues = native.ptr(ev)
     r.residuals = native.ptr(rs)
@@ -72,7 +76,7 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
         out["alpha"] = ta[:r.iterations].copy()
         out["beta"] = tb[:r.iterations].copy()
     if vec is not None:
-        out["vectors"] = vec[:n]
+        out["vectors"] = vec if vectors_out is not None else vec[:n]
     return out
 
 
@@ -167,15 +171,21 @@ def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
 def refine_eigenpairs(H, X, eigenvalues, tol=1e-8, hnorm=40.0, depth=6,
                       max_rounds=4):
     """FP64 refinement of approximate eigenpairs (nh_refine_eigenpairs):
-    block Krylov [X, H X, ..., H^depth X] and Rayleigh-Ritz per round until
-    every ||H x - e x|| <= tol * hnorm.  X: (k, N) host array; returns the
-    refined values, vectors and residuals."""
+    block Krylov [X, H X, ..., H^depth X] and refined Ritz vectors per round
+    until every ||H x - e x|| <= tol * hnorm.  X: (k, N) host array (a
+    refined copy is returned) or native.DeviceArray of shape (>= k, N)
+    (refined in place, k = len(eigenvalues))."""
     lib = native.load()
-    Xc = np.array(X, dtype=np.complex128, order="C", copy=True)
-    k = Xc.shape[0]
-    assert Xc.shape == (k, H.n) and 1 <= k <= 16
     ev = np.array(eigenvalues, dtype=np.float64, copy=True)
-    assert ev.shape == (k,)
+    k = ev.shape[0]
+    if isinstance(X, native.DeviceArray):
+        assert X.dtype == np.complex128 and X.shape[0] >= k and X.shape[1] == H.n
+        Xc, kind = X, native.PTR_DEVICE
+    else:
+        Xc = np.array(X, dtype=np.complex128, order="C", copy=True)
+        assert Xc.shape == (k, H.n)
+        kind = native.PTR_HOST
+    assert 1 <= k <= 16
     rs = np.zeros(k)
     o = native.NHRefineOpts()
     o.depth, o.max_rounds = int(depth), int(max_rounds)
@@ -183,9 +193,8 @@ def refine_eigenpairs(H, X, eigenvalues, tol=1e-8, hnorm=40.0, depth=6,
     rounds, mv = ctypes.c_int32(), ctypes.c_int32()
     t0 = time.perf_counter()
     st = lib.nh_refine_eigenpairs(H.ctx, ctypes.byref(o), k, native.ptr(Xc),
-                                  native.PTR_HOST, native.ptr(ev),
-                                  native.ptr(rs), ctypes.byref(rounds),
-                                  ctypes.byref(mv))
+                                  kind, native.ptr(ev), native.ptr(rs),
+                                  ctypes.byref(rounds), ctypes.byref(mv))
     if st not in (native.NH_OK, 5):
         native.check(st)
     return {"eigenvalues": ev, "vectors": Xc, "residuals": rs,
@@ -208,6 +217,10 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     the learned start block."""
     from . import native as nt
     prec0 = H.precision
+    cap = num_vb + num_cb
+    # the eigenvectors stay in device memory from the FP32 run through the
+    # refinement; only the requested output goes to the host
+    dv = nt.DeviceArray(H, (cap, H.n))
     try:
         H.set_precision(nt.PREC_FP32)
         r1 = eigensolve_lanczos(
@@ -215,15 +228,15 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
             erange_cb=erange_cb, max_iter=max_iter,
             tol=tol if tol32 is None else tol32, start_iter=start_iter,
             skip=skip, start_vector=start_vector, want_vectors=True,
-            return_vectors=True, capacity=num_vb + num_cb, stop_rule=stop_rule)
+            capacity=cap, stop_rule=stop_rule, vectors_out=dv)
         H.set_precision(nt.PREC_FP64)
         n = len(r1["eigenvalues"])
         if n == 0:
-            rf = {"eigenvalues": np.zeros(0), "vectors": np.zeros((0, H.n)),
+            rf = {"eigenvalues": np.zeros(0), "vectors": None,
                   "residuals": np.zeros(0), "rounds": 0, "matvecs": 0,
                   "seconds": 0.0, "converged": False}
         else:
-            rf = refine_eigenpairs(H, r1["vectors"], r1["eigenvalues"], tol=tol,
+            rf = refine_eigenpairs(H, dv, r1["eigenvalues"], tol=tol,
                                    hnorm=hnorm, depth=refine_depth,
                                    max_rounds=refine_rounds)
     finally:
@@ -243,7 +256,7 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
            "refine": {k: rf[k] for k in ("rounds", "matvecs", "seconds",
                                          "converged")}}
     if return_vectors:
-        out["vectors"] = rf["vectors"][order]
+        out["vectors"] = dv.download()[:len(order)][order] if n else np.zeros((0, H.n))
     return out
 
 

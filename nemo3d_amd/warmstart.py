@@ -11,6 +11,7 @@ span each level's two-dimensional subspace.
     python -m nemo3d_amd.warmstart train --dots 16 --out FILE.npz
 """
 import argparse
+import ctypes
 import json
 import time
 
@@ -135,13 +136,41 @@ def load(owner, path):
     return net, json.loads(str(d["meta"]))
 
 
-def start_block(net, H, n_states=learned.N_STATES):
-    """Predicted start block (host, n_states x N) for H's structure,
-    orthonormalised columns."""
-    net.bind(H)
+def structure_inputs(H):
+    """Grid map and per-atom features of H's structure (computed once per
+    structure, like the Hamiltonian itself)."""
     g = learned.GridMap(H.model.structure)
-    f = learned.features(H.model, g)
-    return net.start_block(f, g, n_states)
+    return g, learned.features(H.model, g)
+
+
+def start_block(net, H, n_states=learned.N_STATES, inputs=None, device=False):
+    """Predicted start block (n_states x N) for H's structure, orthonormal
+    columns; a native.DeviceArray when device=True."""
+    net.bind(H)
+    g, f = inputs if inputs is not None else structure_inputs(H)
+    return net.start_block(f, g, n_states, device=device)
+
+
+def start_vector_device(H, block):
+    """start_vector of a device block (native.DeviceArray, n_states x N): the
+    sum of the normalised columns formed on the device, returned on the host
+    (N values).  The block's first column is overwritten."""
+    lib = native.load()
+    k, n = (int(v) for v in block.shape)
+    base = int(block.ptr.value)
+    col = [ctypes.c_void_p(base + c * n * 16) for c in range(k)]
+    one, zero = native.NHCplx(1.0, 0.0), native.NHCplx(0.0, 0.0)
+    for c in range(k):
+        nrm = ctypes.c_double()
+        native.check(lib.nh_nrm2(H.ctx, n, col[c], ctypes.byref(nrm)))
+        s = native.NHCplx(1.0 / nrm.value, 0.0)
+        if c == 0:  # column 0 <- column 0 / |column 0|
+            native.check(lib.nh_axpby(H.ctx, n, zero, col[0], s, col[0]))
+        else:
+            native.check(lib.nh_axpby(H.ctx, n, s, col[c], one, col[0]))
+    out = np.empty(n, dtype=np.complex128)
+    native.check(lib.nh_memcpy_d2h(H.ctx, native.ptr(out), col[0], n * 16))
+    return out
 
 
 def start_vector(block):

@@ -70,6 +70,11 @@ def test_learned_start_gains_over_random():
                 return m
         return None
 
+    # the device-side start vector equals the host one
+    sv_host = warmstart.start_vector(block)
+    sv_dev = warmstart.start_vector_device(H, warmstart.start_block(net, H, device=True))
+    assert np.linalg.norm(sv_dev - sv_host) <= 1e-12 * np.linalg.norm(sv_host)
+
     m_rand = iterations_to_converge(None)
     m_learn = iterations_to_converge(warmstart.start_vector(block))
     print("iterations to converge: random start", m_rand, "learned start",
