@@ -290,6 +290,19 @@ __device__ __forceinline__ void sk_stage_own(int la, int slot,
     if (c0 + slot < kOnsiteStride / 2) S.os[la][c0 + slot] = o2[c0 + slot];
 }
 
+// Exchange of a double with lane ^ 1 / lane ^ 2 inside the quad through
+// DPP quad permutations (VALU moves; a __shfl_xor is an LDS permute).
+template <int CTRL>
+__device__ __forceinline__ double quad_perm_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned)lo));
+}
+constexpr int kQuadXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+
 // The 4 bonds of `atom` (lane = slot), summed over the quad in FP64 (fixed
 // order); lane `slot` keeps rows r = 4 i + slot in own_re[i] / own_im[i].
 // XT: double2 (the FP64 vector) or float2 (its FP32 copy, R = float).
@@ -334,10 +347,10 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
     for (int c = 0; c < 10; ++c) {
       const int r = s * 10 + c;
       double re = double(br[s][c]), im = double(bi[s][c]);
-      re += __shfl_xor(re, 1, 64);
-      im += __shfl_xor(im, 1, 64);
-      re += __shfl_xor(re, 2, 64);
-      im += __shfl_xor(im, 2, 64);
+      re += quad_perm_f64<kQuadXor1>(re);
+      im += quad_perm_f64<kQuadXor1>(im);
+      re += quad_perm_f64<kQuadXor2>(re);
+      im += quad_perm_f64<kQuadXor2>(im);
       if ((r & 3) == slot) {
         own_re[r >> 2] = re;
         own_im[r >> 2] = im;
