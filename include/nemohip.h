@@ -242,6 +242,31 @@ typedef struct nh_tracemin_opts {
 nh_status nh_eigensolve_tracemin(nh_context *ctx, const nh_tracemin_opts *opts,
                                  nh_eig_result *res);
 
+/* PC-TRACEMIN options (PCTracemin deck block: NumEigVal, WantEigRange,
+ * AllEigRange, ChebyDeg, Tolerance, maxi; ctracemin_driver.c:13-40). */
+typedef struct nh_pctracemin_opts {
+  int32_t num_eig;            /* NumEigVal: eigenpairs wanted in want_range */
+  int32_t block_size;         /* block (num_eig..16), 0: num_eig + 4 */
+  double want_range[2];       /* WantEigRange [a, b] */
+  double all_range[2];        /* AllEigRange: bounds of the whole spectrum */
+  int32_t cheb_degree;        /* ChebyDeg k of T_k(1 - gamma (H-a)(H-b)) */
+  double tolerance;           /* ||H x - lambda x|| <= tolerance * hnorm */
+  double hnorm;               /* scale of the test; <= 0: max |AllEigRange| */
+  int32_t max_outer;          /* maxi */
+  uint64_t seed;
+  const nh_cplx *start_block; /* optional host block [block_size][N] */
+} nh_pctracemin_opts;
+
+/* PC-TRACEMIN (ctracemin_driver.c:10-..., map_chebyshev_shift /
+ * zgtracemin, pctracemin_functions.f90:154-332): subspace iteration on
+ * A = T_k(q(H)) + 2I, q(x) = 1 - gamma (x-a)(x-b), gamma from the spectrum
+ * bounds as in the reference, Rayleigh-Ritz of H on the block and FP64
+ * residuals; returns the num_eig Ritz pairs in [a, b] (both members of a
+ * Kramers pair count). */
+nh_status nh_eigensolve_pctracemin(nh_context *ctx,
+                                   const nh_pctracemin_opts *opts,
+                                   nh_eig_result *res);
+
 /* FP64 refinement of approximate eigenpairs, the residual-refinement stage
  * of the mixed-precision solver (Ritz pairs of a run with FP32 products
  * are eigenpairs of the FP32-rounded operator).  Each round builds a block

@@ -142,12 +142,13 @@ __global__ void k_axpy_r(int64_t n, const double *__restrict__ x, double t,
     y[k] = x[k] + t * d[k];
 }
 
-// d = -g + bet d
+// d = -g + bet d; bet == 0 does not read d (a fresh direction may sit in
+// memory holding anything, and 0 * NaN is NaN)
 __global__ void k_cg_dir(int64_t n, const double *__restrict__ g, double bet,
                          double *__restrict__ d) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x)
-    d[k] = -g[k] + bet * d[k];
+    d[k] = bet == 0.0 ? -g[k] : -g[k] + bet * d[k];
 }
 
 class Vff {
@@ -162,6 +163,7 @@ class Vff {
   nh_status alloc(double **p, size_t n) {
     NH_HIP(hipMalloc(p, n * sizeof(double)));
     mem_.push_back(*p);
+    NH_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), c_->stream));
     return NH_OK;
   }
   nh_status grad(const double *u, double *gr, double *energy) {

@@ -87,6 +87,21 @@ class NHBLanczosOpts(ctypes.Structure):
     ]
 
 
+class NHPCTraceminOpts(ctypes.Structure):
+    _fields_ = [
+        ("num_eig", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("want_range", ctypes.c_double * 2),
+        ("all_range", ctypes.c_double * 2),
+        ("cheb_degree", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("hnorm", ctypes.c_double),
+        ("max_outer", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("start_block", ctypes.c_void_p),
+    ]
+
+
 class NHRefineOpts(ctypes.Structure):
     _fields_ = [
         ("depth", ctypes.c_int32),
@@ -152,6 +167,7 @@ SIGNATURES = {
     "nh_eigensolve_lanczos": (ctypes.c_int, [P, P, P]),
     "nh_eigensolve_block_lanczos": (ctypes.c_int, [P, P, P]),
     "nh_eigensolve_tracemin": (ctypes.c_int, [P, P, P]),
+    "nh_eigensolve_pctracemin": (ctypes.c_int, [P, P, P]),
     "nh_refine_eigenpairs": (ctypes.c_int, [P, P, ctypes.c_int32, P, ctypes.c_int,
                                             P, P, P, P]),
     "nh_vff_energy_grad": (ctypes.c_int, [P, P, P, P, P, P, P]),

@@ -168,6 +168,45 @@ def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
     return out
 
 
+def eigensolve_pctracemin(H, num_eig=4, want_range=(0.6, 0.7),
+                          all_range=(-13.0, 40.0), block_size=0, cheb_degree=20,
+                          tol=1e-8, hnorm=40.0, max_outer=200, seed=1,
+                          start_block=None, capacity=16, return_vectors=False):
+    """PC-TRACEMIN (ctracemin_driver, NEMO_3D/nemo3d/src/base/
+    ctracemin_driver.c): subspace iteration on T_k(1 - gamma (H-a)(H-b)) + 2I
+    with Rayleigh-Ritz of H; the num_eig Ritz pairs in want_range."""
+    lib = native.load()
+    o = native.NHPCTraceminOpts()
+    o.num_eig, o.block_size = int(num_eig), int(block_size)
+    o.want_range[0], o.want_range[1] = map(float, want_range)
+    o.all_range[0], o.all_range[1] = map(float, all_range)
+    o.cheb_degree, o.tolerance, o.hnorm = int(cheb_degree), float(tol), float(hnorm)
+    o.max_outer, o.seed = int(max_outer), int(seed)
+    p = block_size if block_size > 0 else min(16, num_eig + 4)
+    if start_block is not None:
+        sb = np.ascontiguousarray(start_block, dtype=np.complex128)
+        assert sb.shape == (p, H.n)
+        o.start_block = native.ptr(sb)
+    ev = np.zeros(capacity)
+    rs = np.zeros(capacity)
+    vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
+    r = native.NHEigResult()
+    r.eigenvalues = native.ptr(ev)
+    r.residuals = native.ptr(rs)
+    r.eigenvectors = native.ptr(vec) if vec is not None else None
+    r.capacity = capacity
+    st = lib.nh_eigensolve_pctracemin(H.ctx, ctypes.byref(o), ctypes.byref(r))
+    if st not in (native.NH_OK, 5):
+        native.check(st)
+    n = r.n_found
+    out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
+           "iterations": r.iterations, "matvecs": r.matvecs,
+           "seconds": r.seconds, "converged": st == native.NH_OK}
+    if vec is not None:
+        out["vectors"] = vec[:n]
+    return out
+
+
 def refine_eigenpairs(H, X, eigenvalues, tol=1e-8, hnorm=40.0, depth=6,
                       max_rounds=4):
     """FP64 refinement of approximate eigenpairs (nh_refine_eigenpairs):

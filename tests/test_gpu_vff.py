@@ -2,7 +2,8 @@
 import numpy as np
 import pytest
 
-from nemo3d_amd import materials, model, structure, vff
+import _models
+from nemo3d_amd import materials, model, native, structure, vff
 from oracle import vff_oracle
 
 pytestmark = pytest.mark.gpu
@@ -76,3 +77,23 @@ def test_relaxation_of_inas_inclusion():
                 isin.append(s.species[i] == 2)
     ln, isin = np.array(ln), np.array(isin)
     assert ln[isin].mean() > ln[~isin].mean() + 0.003
+
+
+def test_relax_after_nan_filled_device_memory():
+    """Relaxation starts from fresh search directions whatever the recycled
+    device memory holds: device buffers are filled with NaN and released
+    first (the first conjugate-gradient direction once read memory it had
+    not written, 0 * NaN), then the relaxation must stay finite and match a
+    relaxation of the same structure in clean memory."""
+    M1 = _models.gaas_box(cells=(3, 3, 3), strain_nm=0.02, seed=41)
+    ref = vff.relax(M1)
+    M2 = _models.gaas_box(cells=(3, 3, 3), strain_nm=0.02, seed=41)
+    from nemo3d_amd.hamiltonian import Hamiltonian
+    owner = Hamiltonian(M2, layout=None)
+    junk = [native.DeviceArray.from_host(owner, np.full(M2.n_atoms * 3 * 8, np.nan))
+            for _ in range(8)]
+    del junk
+    info = vff.relax(M2)
+    assert np.isfinite(M2.structure.disp).all()
+    assert info["iterations"] == ref["iterations"]
+    assert np.abs(M2.structure.disp - M1.structure.disp).max() < 1e-12
