@@ -13,7 +13,9 @@
 // lanczos.c:764-996).  Eigenvectors come from a second pass over the same
 // recurrence (eigenvectorsLanczos, lanczos.c:1364-1689): the Lanczos
 // vectors land in a ring of kRing slots and every kRing iterations one
-// kernel adds Ring * Z to the eigenvector block.
+// kernel adds Ring * Z to the eigenvector block.  The second pass reuses
+// alpha and beta, so with the SK layout its H*v kernel also applies the
+// three-term step (no separate update kernel, no reductions).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -107,15 +109,39 @@ struct LzBlockSum<true> {
   }
 };
 
-template <int NSP, typename R, bool QUAD>
-__global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
+// One element of the three-term step y - alpha q - beta_prev q_prev with
+// q_prev = r_prev * (1 / beta_prev2); rp null on the first step.  The
+// operations are spelled out (explicit fma, y kept opaque) so that the
+// separate update kernel and the H*v kernel of the eigenvector pass, which
+// fuses the step, round identically: the second pass must retrace the first
+// pass's trajectory bit for bit.
+__device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
+                                                 const double2 *rp, double al,
+                                                 double be, double invp) {
+  asm volatile("" : "+v"(y.x), "+v"(y.y));
+  if (rp) {
+    const double2 r = *rp;
+    const double2 p = make_double2(r.x * invp, r.y * invp);
+    y.x = y.x - __builtin_fma(al, q.x, be * p.x);
+    y.y = y.y - __builtin_fma(al, q.y, be * p.y);
+  } else {
+    y.x = __builtin_fma(-al, q.x, y.x);
+    y.y = __builtin_fma(-al, q.y, y.y);
+  }
+  return y;
+}
+
+template <int NSP, typename R, bool QUAD, bool FUSE>
+__global__ __launch_bounds__(kSkThreads, 2) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, int ppx,
     const double *__restrict__ onsite, const R *__restrict__ sk,
     const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
     const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
     double2 *__restrict__ Q, double2 *__restrict__ Y,
-    const double *__restrict__ beta_prev, double *__restrict__ part) {
+    const double *__restrict__ beta_prev, double *__restrict__ part,
+    const double *__restrict__ alpha, const double2 *__restrict__ Rp,
+    const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
   constexpr int NB = NSP * 10;
   constexpr int NO = (NB + 3) / 4;
   const SkBlock wb = sk_block_of(blockIdx.x, ppx);
@@ -126,6 +152,11 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
   const bool active = li < (cat ? nc : na);
   const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
   const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
+  // FUSE (eigenvector pass): alpha is known, so the three-term step of
+  // k_lz_update is applied here and Y receives r_{k+1}
+  const double al = FUSE ? alpha[0] : 0.0;
+  const double be = beta_prev ? beta_prev[0] : 0.0;
+  const double invp = beta_prev2 ? 1.0 / beta_prev2[0] : 1.0;
   double own_re[NO], own_im[NO];
   // FP32 products gather from the FP32 copy of r (the same rounding as
   // converting the FP64 entries on load)
@@ -165,9 +196,14 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
         onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la],
                         r / 10, r % 10, 0.0, re, im);
         const double2 q = make_double2(own_x[i].x * inv, own_x[i].y * inv);
-        const double2 y = make_double2(re * inv, im * inv);
-        if (Q) Q[(int64_t)atom * NB + r] = q;
-        Y[(int64_t)atom * NB + r] = y;
+        const int64_t e = (int64_t)atom * NB + r;
+        double2 y = make_double2(re * inv, im * inv);
+        if constexpr (FUSE) {
+          y = lz_three_term(y, q, Rp ? Rp + e : nullptr, al, be, invp);
+          if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+        }
+        if (Q) Q[e] = q;
+        Y[e] = y;
         local += q.x * y.x + q.y * y.y;
       }
     }
@@ -182,9 +218,14 @@ __global__ __launch_bounds__(kSkThreads) void k_lz_hv_sk(
       onsite_row<NSP>(os, xa, r / 10, r % 10, 0.0, re, im);
       const double2 xr = xa[r];
       const double2 q = make_double2(xr.x * inv, xr.y * inv);
-      const double2 y = make_double2(re * inv, im * inv);
-      if (Q) Q[(int64_t)atom * NB + r] = q;
-      Y[(int64_t)atom * NB + r] = y;
+      const int64_t e = (int64_t)atom * NB + r;
+      double2 y = make_double2(re * inv, im * inv);
+      if constexpr (FUSE) {
+        y = lz_three_term(y, q, Rp ? Rp + e : nullptr, al, be, invp);
+        if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+      }
+      if (Q) Q[e] = q;
+      Y[e] = y;
       local += q.x * y.x + q.y * y.y;
     }
   }
@@ -213,18 +254,9 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_update(
   double s = 0.0, z = 0.0;
   for (int64_t k = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * kRedThreads) {
-    double2 y = Y[k];
     const double2 r = Rk[k];
     const double2 q = make_double2(r.x * inv, r.y * inv);
-    if (Rp) {
-      const double2 rp = Rp[k];
-      const double2 p = make_double2(rp.x * invp, rp.y * invp);
-      y.x -= al * q.x + be * p.x;
-      y.y -= al * q.y + be * p.y;
-    } else {
-      y.x -= al * q.x;
-      y.y -= al * q.y;
-    }
+    const double2 y = lz_three_term(Y[k], q, Rp ? Rp + k : nullptr, al, be, invp);
     Y[k] = y;
     if (Y32) Y32[k] = make_float2((float)y.x, (float)y.y);
     s += y.x * y.x + y.y * y.y;
@@ -460,6 +492,8 @@ nh_status LanczosRun::iterate(int k, bool red) {
   const double *bprev2 = k > 1 ? beta_ + (k - 2) : nullptr;
   double *part = red ? part_ : nullptr;
   const bool timed = red && (k % kTimeEvery) == 0;
+  // the eigenvector pass knows alpha_k: the SK kernel applies the update
+  const bool fuse = !red && c->layout == NH_LAYOUT_SK;
   std::pair<hipEvent_t, hipEvent_t> evp;
   if (timed) {
     NH_HIP(hipEventCreate(&evp.first));
@@ -477,11 +511,17 @@ nh_status LanczosRun::iterate(int k, bool red) {
                          dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
                          BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
   } while (0)
+#define LZ_SK2(NSP, R, SKP, QUAD, FUSE)                                         \
+  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE>), dim3(gridA),             \
+                     dim3(kSkThreads), 0, c->stream, c->n_cations, c->cations,  \
+                     c->n_anions, c->anions, ppx, c->onsite, SKP, c->nbr,       \
+                     c->bond_of, R_, R32_, Q, Y_, bprev, part, alpha_ + k,      \
+                     k > 0 ? Rm1_ : nullptr, bprev2, Y32_)
 #define LZ_SK1(NSP, R, SKP, QUAD)                                               \
-  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD>), dim3(gridA), dim3(kSkThreads),  \
-                     0, c->stream, c->n_cations, c->cations, c->n_anions,       \
-                     c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of, R_,    \
-                     R32_, Q, Y_, bprev, part)
+  do {                                                                          \
+    if (fuse) LZ_SK2(NSP, R, SKP, QUAD, true);                                  \
+    else LZ_SK2(NSP, R, SKP, QUAD, false);                                      \
+  } while (0)
 #define LZ_SK(NSP, R, SKP)                                                      \
   do {                                                                          \
     if (use_staged(c)) LZ_SK1(NSP, R, SKP, true);                                \
@@ -505,6 +545,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #undef LZ_HV
 #undef LZ_SK
 #undef LZ_SK1
+#undef LZ_SK2
   if (timed) {
     NH_HIP(hipEventRecord(evp.second, c->stream));
     ev_.push_back(evp);
@@ -513,9 +554,10 @@ nh_status LanczosRun::iterate(int k, bool red) {
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        gridA, 1, part_, alpha_ + k, 0);
   const int nb = reduce_blocks(n_);
-  hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
-                     n_, Y_, R_, k > 0 ? Rm1_ : nullptr, alpha_ + k, bprev,
-                     bprev2, part, Y32_);
+  if (!fuse)
+    hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
+                       n_, Y_, R_, k > 0 ? Rm1_ : nullptr, alpha_ + k, bprev,
+                       bprev2, part, Y32_);
   if (red)
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        nb, 2, part_, beta_ + k, 1);

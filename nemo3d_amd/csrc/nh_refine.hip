@@ -26,7 +26,8 @@ constexpr int kTile = 8;  // gram tile (k_gram<8>)
 
 class Refine {
  public:
-  Refine(nh_context *c, int k, int depth) : c_(c), k_(k), d_(depth) {
+  Refine(nh_context *c, int k, int depth, double tol, double hnorm)
+      : c_(c), k_(k), d_(depth), tol_(tol), hnorm_(hnorm) {
     n_ = c->n;
     kp_ = (k + kTile - 1) / kTile * kTile;
   }
@@ -57,6 +58,12 @@ class Refine {
 
   nh_context *c_;
   int k_, d_, kp_ = 0;
+  double tol_, hnorm_;
+
+ public:
+  int basis_blocks_ = 0;  // blocks used, summed over rounds
+
+ private:
   int64_t n_ = 0;
   std::vector<void *> mem_;
   double2 *V_ = nullptr, *W_ = nullptr, *X_ = nullptr, *S_ = nullptr;
@@ -165,7 +172,62 @@ nh_status Refine::round(std::vector<double> &target, std::vector<double> &resid,
   std::vector<cd> R, R0;
   NH_TRY(cholqr(X_, blk(0), R0, ok));  // X = V_0 R0
   NH_CHECK(ok, NH_ERR_ARG, "refine: start vectors are linearly dependent");
+  // Arnoldi matrix and Gram factor of the remainder for a basis of nb
+  // blocks; refined Ritz vectors (y_j), values and the residual norms
+  // ||(H - e) V y|| they give (exact up to rounding)
+  std::vector<cd> Rw;
+  auto small = [&](int nb, int outers, std::vector<cd> &M,
+                   std::vector<double> &theta, std::vector<double> &sigma) {
+    std::vector<int> idx;
+    for (int b = 0; b < nb; ++b)
+      for (int cc = 0; cc < k; ++cc) idx.push_back(b * kp + cc);
+    const int m = (int)idx.size();
+    std::vector<cd> Ta((size_t)m * m), Th((size_t)m * m);
+    for (int r = 0; r < m; ++r)
+      for (int cc = 0; cc < m; ++cc) {
+        Ta[(size_t)r * m + cc] = T[(size_t)idx[r] * mp + idx[cc]];
+        Th[(size_t)r * m + cc] = 0.5 * (T[(size_t)idx[r] * mp + idx[cc]] +
+                                        std::conj(T[(size_t)idx[cc] * mp + idx[r]]));
+      }
+    M.assign((size_t)nb * kp * k, 0.0);
+    theta.assign(k, 0.0);
+    sigma.assign(k, 0.0);
+    for (int j = 0; j < k; ++j) {
+      std::vector<cd> y(m, 0.0);
+      for (int a = 0; a < k; ++a) y[a] = R0[a * k + j];
+      double th = target[j];
+      for (int outer = 0; outer < outers; ++outer) {
+        refined_ritz(m, k, Ta, Rw, th, y);
+        cd rq = 0.0;
+        for (int r = 0; r < m; ++r) {
+          cd t = 0.0;
+          for (int cc = 0; cc < m; ++cc) t += Th[(size_t)r * m + cc] * y[cc];
+          rq += std::conj(y[r]) * t;
+        }
+        th = rq.real();
+      }
+      // ||(T - th) y||^2 + ||Rw y_last||^2
+      double s2 = 0.0;
+      for (int r = 0; r < m; ++r) {
+        cd t = -th * y[r];
+        for (int cc = 0; cc < m; ++cc) t += Ta[(size_t)r * m + cc] * y[cc];
+        s2 += std::norm(t);
+      }
+      for (int r = 0; r < k; ++r) {
+        cd t = 0.0;
+        for (int cc = r; cc < k; ++cc) t += Rw[r * k + cc] * y[m - k + cc];
+        s2 += std::norm(t);
+      }
+      theta[j] = th;
+      sigma[j] = std::sqrt(s2);
+      for (int r = 0; r < m; ++r) M[(size_t)idx[r] * k + j] = y[r];
+    }
+  };
+  // the basis grows one block at a time; from 4 blocks on (every second
+  // step) the refined residuals decide whether the next block is needed
   int nb = 1;  // blocks in the basis
+  std::vector<cd> M;
+  std::vector<double> theta, sigma;
   for (int i = 0; i <= d_; ++i) {
     NH_TRY(launch_hv(c_, 0.0, reinterpret_cast<const nh_cplx *>(blk(i)),
                      reinterpret_cast<nh_cplx *>(W_), k, n));
@@ -181,7 +243,25 @@ nh_status Refine::round(std::vector<double> &target, std::vector<double> &resid,
         }
       NH_TRY(gemm(V_, rows, W_, k, Cm, 1.0));
     }
+    // H V = V T + W E^T with W (remainder of block i) orthogonal to V: its
+    // Gram factor Rw enters the residual norms of combinations of V
+    std::vector<cd> G;
+    NH_TRY(gram(W_, kp, W_, kp, G));
+    Rw.assign((size_t)k * k, 0.0);
+    for (int a = 0; a < k; ++a)
+      for (int b = 0; b < k; ++b) Rw[a * k + b] = G[(size_t)a * kp + b];
+    double gtr = 0.0;
+    for (int a = 0; a < k; ++a) gtr += Rw[a * k + a].real();
+    for (int a = 0; a < k; ++a) Rw[a * k + a] += 1e-30 * gtr + 1e-300;
+    if (!cholesky_upper(k, Rw)) std::fill(Rw.begin(), Rw.end(), cd(0.0));
     if (i == d_) break;
+    if (i >= 3 && (i & 1) == 1) {
+      small(nb, 1, M, theta, sigma);
+      bool enough = true;
+      for (int j = 0; j < k; ++j)
+        if (!(sigma[j] <= 0.8 * tol_ * hnorm_)) enough = false;
+      if (enough) break;
+    }
     NH_TRY(cholqr(W_, blk(i + 1), R, ok));
     if (!ok) break;  // Krylov space exhausted: W holds the tiny remainder
     for (int a = 0; a < k; ++a)
@@ -189,50 +269,13 @@ nh_status Refine::round(std::vector<double> &target, std::vector<double> &resid,
         T[(size_t)((i + 1) * kp + a) * mp + i * kp + b] = R[a * k + b];
     ++nb;
   }
-  // H V = V T + W E^T with W (remainder of the last block) orthogonal to V:
-  // its Gram factor Rw enters the residual norms of combinations of V
-  std::vector<cd> G, Rw((size_t)k * k, 0.0);
-  NH_TRY(gram(W_, kp, W_, kp, G));
-  for (int a = 0; a < k; ++a)
-    for (int b = 0; b < k; ++b) Rw[a * k + b] = G[(size_t)a * kp + b];
-  double gtr = 0.0;
-  for (int a = 0; a < k; ++a) gtr += Rw[a * k + a].real();
-  for (int a = 0; a < k; ++a) Rw[a * k + a] += 1e-30 * gtr + 1e-300;
-  if (!cholesky_upper(k, Rw)) std::fill(Rw.begin(), Rw.end(), cd(0.0));
-  // compress to valid indices: the Arnoldi matrix and its Hermitian part
-  std::vector<int> idx;
-  for (int b = 0; b < nb; ++b)
-    for (int cc = 0; cc < k; ++cc) idx.push_back(b * kp + cc);
-  const int m = (int)idx.size();
-  std::vector<cd> Ta((size_t)m * m), Th((size_t)m * m);
-  for (int r = 0; r < m; ++r)
-    for (int cc = 0; cc < m; ++cc) {
-      Ta[(size_t)r * m + cc] = T[(size_t)idx[r] * mp + idx[cc]];
-      Th[(size_t)r * m + cc] = 0.5 * (T[(size_t)idx[r] * mp + idx[cc]] +
-                                      std::conj(T[(size_t)idx[cc] * mp + idx[r]]));
-    }
+  basis_blocks_ += nb;
   // refined Ritz vector per eigenvalue: the unit combination of the basis
   // with the smallest ||(H - e) x|| (interior eigenvalues: plain Ritz
   // vectors of a growing basis pick up spurious mixtures), e updated to its
   // Rayleigh quotient, twice; started from the current vector (V_0 R0 e_j)
-  std::vector<cd> M((size_t)nb * kp * k, 0.0);
-  for (int j = 0; j < k; ++j) {
-    std::vector<cd> y(m, 0.0);
-    for (int a = 0; a < k; ++a) y[a] = R0[a * k + j];
-    double th = target[j];
-    for (int outer = 0; outer < 2; ++outer) {
-      refined_ritz(m, k, Ta, Rw, th, y);
-      cd rq = 0.0;
-      for (int r = 0; r < m; ++r) {
-        cd t = 0.0;
-        for (int cc = 0; cc < m; ++cc) t += Th[(size_t)r * m + cc] * y[cc];
-        rq += std::conj(y[r]) * t;
-      }
-      th = rq.real();
-    }
-    target[j] = th;
-    for (int r = 0; r < m; ++r) M[(size_t)idx[r] * k + j] = y[r];
-  }
+  small(nb, 2, M, theta, sigma);
+  for (int j = 0; j < k; ++j) target[j] = theta[j];
   // X = V Y (padded rows zero)
   NH_TRY(gemm(V_, nb * kp, X_, k, M, 0.0));
   // residuals ||H x - e x|| (explicit difference, no cancellation)
@@ -288,7 +331,7 @@ extern "C" nh_status nh_refine_eigenpairs(nh_context *ctx,
   NH_CHECK(ctx->precision == NH_PREC_FP64, NH_ERR_STATE,
            "refine: the operator must be FP64 (nh_ham_set_precision)");
   NH_HIP(hipSetDevice(ctx->device));
-  Refine rf(ctx, k, o->depth);
+  Refine rf(ctx, k, o->depth, o->tolerance, o->hnorm);
   NH_TRY(rf.setup());
   const size_t bytes = (size_t)k * ctx->n * sizeof(double2);
   NH_HIP(hipMemcpyAsync(rf.X(), X, bytes,
