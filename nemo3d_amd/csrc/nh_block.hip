@@ -90,8 +90,8 @@ nh_status BlockLanczos::gram(const double2 *X, const double2 *Y,
       hipLaunchKernelGGL((k_gram<P>), dim3(nparts_), dim3(256), 0, c_->stream,
                          n_, q, X + (size_t)a0 * n_, n_, Y + (size_t)b0 * n_,
                          n_, part_);
-      hipLaunchKernelGGL(k_sum_cols, dim3(4), dim3(256), 0, c_->stream,
-                         nparts_, 2 * P * P, part_, sum_);
+      hipLaunchKernelGGL(k_sum_cols, dim3(sum_cols_grid(2 * P * P)), dim3(256), 0,
+                         c_->stream, nparts_, 2 * P * P, part_, sum_);
       NH_HIP(hipGetLastError());
       NH_HIP(hipMemcpyAsync(h.data(), sum_, h.size() * sizeof(double),
                             hipMemcpyDeviceToHost, c_->stream));

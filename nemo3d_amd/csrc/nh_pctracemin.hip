@@ -194,7 +194,7 @@ nh_status PCTracemin::run(nh_eig_result *res) {
                        n_, p_, Y_, n_, W1_, n_, dab_);
     hipLaunchKernelGGL((k_coldot<16>), dim3(nparts_), dim3(256), 0, c->stream,
                        n_, p_, W1_, n_, W1_, n_, part_);
-    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(256), 0, c->stream, nparts_,
+    hipLaunchKernelGGL(k_sum_cols, dim3(32), dim3(256), 0, c->stream, nparts_,
                        32, part_, sum_);
     NH_HIP(hipGetLastError());
     std::vector<double> h(32);

@@ -16,20 +16,21 @@
 #include <vector>
 
 #include "nh_blockla.h"
+#include "nh_blockops.h"
 #include "nh_internal.h"
 #include "nh_vector.h"
 
 namespace nh {
 namespace {
 
-constexpr int kTile = 8;  // gram tile (k_gram<8>)
+constexpr int kTile = 8;  // block padding: columns per Gram tile
 
 class Refine {
  public:
   Refine(nh_context *c, int k, int depth, double tol, double hnorm)
-      : c_(c), k_(k), d_(depth), tol_(tol), hnorm_(hnorm) {
+      : c_(c), k_(k), d_(depth), kp_((k + kTile - 1) / kTile * kTile),
+        tol_(tol), hnorm_(hnorm), ops_(c, kp_, (depth + 1) * kp_) {
     n_ = c->n;
-    kp_ = (k + kTile - 1) / kTile * kTile;
   }
   ~Refine() {
     for (void *q : mem_) hipFree(q);
@@ -47,18 +48,25 @@ class Refine {
     return NH_OK;
   }
   double2 *blk(int i) { return V_ + (size_t)i * kp_ * n_; }
-  // G (qx x qy, row-major) = X^H Y; qx, qy multiples of kTile
+  // G (qx x qy, row-major) = X^H Y
   nh_status gram(const double2 *X, int qx, const double2 *Y, int qy,
-                 std::vector<cd> &G);
-  // Y[:, 0..qy) = beta Y + X M, M (qx x qy, row-major), qy <= 16
+                 std::vector<cd> &G) {
+    return ops_.gram(X, qx, Y, qy, G);
+  }
+  // Y[:, 0..qy) = beta Y + X M, M (qx x qy, row-major)
   nh_status gemm(const double2 *X, int qx, double2 *Y, int qy,
-                 const std::vector<cd> &M, double beta);
+                 const std::vector<cd> &M, double beta) {
+    return ops_.gemm(X, qx, Y, qy, M, beta);
+  }
   // Cholesky QR twice of the k columns of W into Q; R = R2 R1 (k x k)
-  nh_status cholqr(double2 *W, double2 *Q, std::vector<cd> &R, bool &ok);
+  nh_status cholqr(double2 *W, double2 *Q, std::vector<cd> &R, bool &ok) {
+    return ops_.cholqr(W, Q, k_, R, ok);
+  }
 
   nh_context *c_;
-  int k_, d_, kp_ = 0;
+  int k_, d_, kp_;
   double tol_, hnorm_;
+  BlockOps ops_;
 
  public:
   int basis_blocks_ = 0;  // blocks used, summed over rounds
@@ -66,98 +74,20 @@ class Refine {
  private:
   int64_t n_ = 0;
   std::vector<void *> mem_;
-  double2 *V_ = nullptr, *W_ = nullptr, *X_ = nullptr, *S_ = nullptr;
+  double2 *V_ = nullptr, *W_ = nullptr, *X_ = nullptr;
   double *part_ = nullptr, *sum_ = nullptr, *dM_ = nullptr;
   int nparts_ = 0;
 };
 
 nh_status Refine::setup() {
+  NH_TRY(ops_.setup());
   nparts_ = reduce_blocks(n_);
   NH_TRY(alloc((void **)&V_, (size_t)(d_ + 1) * kp_ * n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&W_, (size_t)kp_ * n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&X_, (size_t)kp_ * n_ * sizeof(double2)));
-  NH_TRY(alloc((void **)&S_, (size_t)kp_ * n_ * sizeof(double2)));
-  NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * kTile * kTile * sizeof(double)));
-  NH_TRY(alloc((void **)&sum_, 2 * kTile * kTile * sizeof(double)));
-  NH_TRY(alloc((void **)&dM_, 2 * 16 * 16 * sizeof(double)));
-  return NH_OK;
-}
-
-nh_status Refine::gram(const double2 *X, int qx, const double2 *Y, int qy,
-                       std::vector<cd> &G) {
-  G.assign((size_t)qx * qy, 0.0);
-  std::vector<double> h(2 * kTile * kTile);
-  for (int a0 = 0; a0 < qx; a0 += kTile)
-    for (int b0 = 0; b0 < qy; b0 += kTile) {
-      hipLaunchKernelGGL((k_gram<kTile>), dim3(nparts_), dim3(256), 0,
-                         c_->stream, n_, kTile, X + (size_t)a0 * n_, n_,
-                         Y + (size_t)b0 * n_, n_, part_);
-      hipLaunchKernelGGL(k_sum_cols, dim3(4), dim3(256), 0, c_->stream,
-                         nparts_, 2 * kTile * kTile, part_, sum_);
-      NH_HIP(hipGetLastError());
-      NH_HIP(hipMemcpyAsync(h.data(), sum_, h.size() * sizeof(double),
-                            hipMemcpyDeviceToHost, c_->stream));
-      NH_HIP(hipStreamSynchronize(c_->stream));
-      for (int a = 0; a < kTile; ++a)
-        for (int b = 0; b < kTile; ++b)
-          G[(size_t)(a0 + a) * qy + b0 + b] =
-              cd(h[2 * (a * kTile + b)], h[2 * (a * kTile + b) + 1]);
-    }
-  return NH_OK;
-}
-
-nh_status Refine::gemm(const double2 *X, int qx, double2 *Y, int qy,
-                       const std::vector<cd> &M, double beta) {
-  NH_CHECK(qy <= 16, NH_ERR_ARG, "refine: gemm width %d", qy);
-  int64_t g = (n_ + 255) / 256;
-  if (g > 16384) g = 16384;
-  for (int x0 = 0; x0 < qx; x0 += 16) {
-    const int q = std::min(16, qx - x0);
-    std::vector<double> h(2 * q * qy);
-    for (int a = 0; a < q; ++a)
-      for (int b = 0; b < qy; ++b) {
-        h[2 * (a * qy + b)] = M[(size_t)(x0 + a) * qy + b].real();
-        h[2 * (a * qy + b) + 1] = M[(size_t)(x0 + a) * qy + b].imag();
-      }
-    NH_HIP(hipMemcpyAsync(dM_, h.data(), h.size() * sizeof(double),
-                          hipMemcpyHostToDevice, c_->stream));
-    hipLaunchKernelGGL((k_block_gemm<16>), dim3((int)g), dim3(256), 0,
-                       c_->stream, n_, q, qy, X + (size_t)x0 * n_, n_, Y, n_,
-                       dM_, x0 == 0 ? beta : 1.0);
-    NH_HIP(hipGetLastError());
-    // h is released on return: wait for the copy
-    NH_HIP(hipStreamSynchronize(c_->stream));
-  }
-  return NH_OK;
-}
-
-nh_status Refine::cholqr(double2 *W, double2 *Q, std::vector<cd> &R,
-                         bool &ok) {
-  const int k = k_;
-  std::vector<cd> G, F[2];
-  ok = true;
-  for (int pass = 0; pass < 2; ++pass) {
-    double2 *src = pass == 0 ? W : Q;
-    double2 *dst = pass == 0 ? Q : S_;
-    NH_TRY(gram(src, kp_, src, kp_, G));
-    F[pass].assign((size_t)k * k, 0.0);
-    for (int a = 0; a < k; ++a)
-      for (int b = 0; b < k; ++b) F[pass][a * k + b] = G[(size_t)a * kp_ + b];
-    double dmax = 0.0, dmin = 1e300;
-    for (int a = 0; a < k; ++a) {
-      dmax = std::max(dmax, F[pass][a * k + a].real());
-      dmin = std::min(dmin, F[pass][a * k + a].real());
-    }
-    // a column that lies in the span of the basis already: stop expanding
-    if (!(dmin > 1e-24 * std::max(dmax, 1e-300)) || !cholesky_upper(k, F[pass])) {
-      ok = false;
-      return NH_OK;
-    }
-    NH_TRY(gemm(src, k, dst, k, inv_upper(k, F[pass]), 0.0));
-  }
-  NH_HIP(hipMemcpyAsync(Q, S_, (size_t)k * n_ * sizeof(double2),
-                        hipMemcpyDeviceToDevice, c_->stream));
-  R = matmul(k, F[1], F[0]);
+  NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 32 * sizeof(double)));
+  NH_TRY(alloc((void **)&sum_, 32 * sizeof(double)));
+  NH_TRY(alloc((void **)&dM_, 32 * sizeof(double)));
   return NH_OK;
 }
 
@@ -295,7 +225,7 @@ nh_status Refine::round(std::vector<double> &target, std::vector<double> &resid,
                      n, k, X_, n, W_, n, dM_);
   hipLaunchKernelGGL((k_coldot<16>), dim3(nparts_), dim3(256), 0, c_->stream,
                      n, k, W_, n, W_, n, part_);
-  hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(256), 0, c_->stream, nparts_,
+  hipLaunchKernelGGL(k_sum_cols, dim3(32), dim3(256), 0, c_->stream, nparts_,
                      32, part_, sum_);
   NH_HIP(hipGetLastError());
   std::vector<double> h(32);

@@ -63,8 +63,8 @@ class Tracemin {
         hipLaunchKernelGGL((k_gram<P>), dim3(kParts), dim3(256), 0, c_->stream,
                            n_, std::max(qa, qb), X + (size_t)a0 * n_, n_,
                            Y + (size_t)b0 * n_, n_, part_);
-        hipLaunchKernelGGL(k_sum_cols, dim3(4), dim3(256), 0, c_->stream, kParts,
-                           2 * P * P, part_, sum_);
+        hipLaunchKernelGGL(k_sum_cols, dim3(sum_cols_grid(2 * P * P)), dim3(256), 0,
+                           c_->stream, kParts, 2 * P * P, part_, sum_);
         NH_HIP(hipGetLastError());
         NH_HIP(hipMemcpyAsync(h.data(), sum_, h.size() * sizeof(double),
                               hipMemcpyDeviceToHost, c_->stream));
@@ -79,7 +79,7 @@ class Tracemin {
   nh_status coldot(const double2 *X, const double2 *Y, std::vector<cd> &d) {
     hipLaunchKernelGGL((k_coldot<kMaxP>), dim3(kParts), dim3(256), 0,
                        c_->stream, n_, s_, X, n_, Y, n_, part_);
-    hipLaunchKernelGGL(k_sum_cols, dim3(1), dim3(64), 0, c_->stream, kParts,
+    hipLaunchKernelGGL(k_sum_cols, dim3(2 * kMaxP), dim3(64), 0, c_->stream, kParts,
                        2 * kMaxP, part_, sum_);
     NH_HIP(hipGetLastError());
     std::vector<double> h(2 * kMaxP);
