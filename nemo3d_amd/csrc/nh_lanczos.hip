@@ -277,10 +277,12 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_scalar(
   if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(a) : a;
 }
 
-// V[e] += sum_k Z[e][k] Ring[k] for e < ne, k < K.
+// V[e] += sum_k Z[e][k] Ring[k] for e < ne, k < K (ring of FP64 q or of
+// the FP32 copies of r).
+template <typename T>
 __global__ __launch_bounds__(256) void k_lz_accum(
     int64_t n, int ne, int K, const double *__restrict__ Z,
-    const double2 *__restrict__ ring, int64_t ring_ld, double2 *__restrict__ V,
+    const T *__restrict__ ring, int64_t ring_ld, double2 *__restrict__ V,
     int64_t v_ld) {
   __shared__ double zs[kMaxVec * kRing];
   for (int t = threadIdx.x; t < ne * K; t += blockDim.x) zs[t] = Z[t];
@@ -289,8 +291,13 @@ __global__ __launch_bounds__(256) void k_lz_accum(
        i += (int64_t)gridDim.x * blockDim.x) {
     double2 q[kRing];
 #pragma unroll
-    for (int k = 0; k < kRing; ++k)
-      q[k] = k < K ? ring[(int64_t)k * ring_ld + i] : make_double2(0.0, 0.0);
+    for (int k = 0; k < kRing; ++k) {
+      q[k] = make_double2(0.0, 0.0);
+      if (k < K) {
+        const T v = ring[(int64_t)k * ring_ld + i];
+        q[k] = make_double2(v.x, v.y);
+      }
+    }
     for (int e = 0; e < ne; ++e) {
       double2 v = V[(int64_t)e * v_ld + i];
 #pragma unroll
@@ -421,7 +428,12 @@ class LanczosRun {
   double2 *R_ = nullptr, *Y_ = nullptr, *Rm1_ = nullptr, *ring_ = nullptr;
   // FP32 copies of r for the FP32 SK products (null otherwise)
   float2 *R32_ = nullptr, *Y32_ = nullptr;
-  nh_status upload_start(const std::vector<nh_cplx> &r0);
+  // eigenvector pass with FP32 products: 2 kRing FP32 copies of r, slot
+  // k % (2 kRing) holding r_k; the gather of step k reads slot k and writes
+  // r_{k+1} to slot k + 1, and the accumulation reads them as its ring
+  // (1 / beta_{k-1} folded into the coefficients)
+  float2 *ring32_ = nullptr;
+  nh_status upload_start(const std::vector<nh_cplx> &r0, float2 *f32);
   double *alpha_ = nullptr, *beta_ = nullptr, *part_ = nullptr;
   int nparts_ = 0;
   std::vector<double> ha_, hb_;
@@ -466,13 +478,13 @@ __global__ __launch_bounds__(256) void k_to_f32(int64_t n,
     y[k] = make_float2((float)x[k].x, (float)x[k].y);
 }
 
-// r <- r0 (and its FP32 copy when the products are FP32)
-nh_status LanczosRun::upload_start(const std::vector<nh_cplx> &r0) {
+// r <- r0 (and its FP32 copy f32 when the products are FP32)
+nh_status LanczosRun::upload_start(const std::vector<nh_cplx> &r0, float2 *f32) {
   NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
                         hipMemcpyHostToDevice, c_->stream));
-  if (R32_) {
+  if (f32) {
     hipLaunchKernelGGL(k_to_f32, dim3(reduce_blocks(n_)), dim3(256), 0,
-                       c_->stream, n_, R_, R32_);
+                       c_->stream, n_, R_, f32);
     NH_HIP(hipGetLastError());
   }
   // the host vector r0 lives on: the copy may still be in flight
@@ -487,7 +499,14 @@ nh_status LanczosRun::iterate(int k, bool red) {
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   // q_k is stored (ring slot) only in the eigenvector pass, which
   // accumulates from it; the update forms q_k, q_{k-1} from r_k, r_{k-1}
-  double2 *Q = red ? nullptr : ring_ + (int64_t)(k % kRing) * n_;
+  double2 *Q = red || ring32_ ? nullptr : ring_ + (int64_t)(k % kRing) * n_;
+  // FP32 copies: gathered (r_k) and written (r_{k+1})
+  const float2 *g32 = R32_;
+  float2 *y32 = Y32_;
+  if (!red && ring32_) {
+    g32 = ring32_ + (int64_t)(k % (2 * kRing)) * n_;
+    y32 = ring32_ + (int64_t)((k + 1) % (2 * kRing)) * n_;
+  }
   const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
   const double *bprev2 = k > 1 ? beta_ + (k - 2) : nullptr;
   double *part = red ? part_ : nullptr;
@@ -515,8 +534,8 @@ nh_status LanczosRun::iterate(int k, bool red) {
   hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE>), dim3(gridA),             \
                      dim3(kSkThreads), 0, c->stream, c->n_cations, c->cations,  \
                      c->n_anions, c->anions, ppx, c->onsite, SKP, c->nbr,       \
-                     c->bond_of, R_, R32_, Q, Y_, bprev, part, alpha_ + k,      \
-                     k > 0 ? Rm1_ : nullptr, bprev2, Y32_)
+                     c->bond_of, R_, g32, Q, Y_, bprev, part, alpha_ + k,       \
+                     k > 0 ? Rm1_ : nullptr, bprev2, y32)
 #define LZ_SK1(NSP, R, SKP, QUAD)                                               \
   do {                                                                          \
     if (fuse) LZ_SK2(NSP, R, SKP, QUAD, true);                                  \
@@ -557,7 +576,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
   if (!fuse)
     hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
                        n_, Y_, R_, k > 0 ? Rm1_ : nullptr, alpha_ + k, bprev,
-                       bprev2, part, Y32_);
+                       bprev2, part, y32);
   if (red)
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        nb, 2, part_, beta_ + k, 1);
@@ -657,7 +676,6 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   NH_TRY(alloc((void **)&R_, n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&Y_, n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&Rm1_, n_ * sizeof(double2)));
-  NH_TRY(alloc((void **)&ring_, (size_t)kRing * n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&alpha_, (size_t)(o->max_iter + 1) * sizeof(double)));
   NH_TRY(alloc((void **)&beta_, (size_t)(o->max_iter + 1) * sizeof(double)));
   nparts_ = std::max((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock,
@@ -692,7 +710,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_TRY(alloc((void **)&R32_, n_ * sizeof(float2)));
     NH_TRY(alloc((void **)&Y32_, n_ * sizeof(float2)));
   }
-  NH_TRY(upload_start(r0));
+  NH_TRY(upload_start(r0, R32_));
 
   // pass 1: eigenvalues
   int m = 0, skip = 0;
@@ -773,7 +791,11 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
     NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRing * sizeof(double)));
     NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
-    NH_TRY(upload_start(r0));
+    if (R32_)
+      NH_TRY(alloc((void **)&ring32_, (size_t)2 * kRing * n_ * sizeof(float2)));
+    else
+      NH_TRY(alloc((void **)&ring_, (size_t)kRing * n_ * sizeof(double2)));
+    NH_TRY(upload_start(r0, ring32_));
     std::vector<double> hz((size_t)kMaxVec * kRing);
     for (int k = 0; k < mmax; ++k) {
       NH_TRY(iterate(k, false));
@@ -782,15 +804,25 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
         const int k0 = k + 1 - ((k % kRing) + 1);
         const int K = k + 1 - k0;
         for (int e = 0; e < ne; ++e)
-          for (int t = 0; t < K; ++t)
-            hz[e * K + t] = k0 + t < pick[e].iter ? z[e][k0 + t] : 0.0;
+          for (int t = 0; t < K; ++t) {
+            const int j = k0 + t;
+            // q_j = r_j / beta_{j-1} for the FP32 ring of r copies
+            const double sc = ring32_ && j > 0 ? 1.0 / hb_[j - 1] : 1.0;
+            hz[e * K + t] = j < pick[e].iter ? z[e][j] * sc : 0.0;
+          }
         NH_HIP(hipMemcpyAsync(dZ, hz.data(), (size_t)ne * K * sizeof(double),
                               hipMemcpyHostToDevice, c->stream));
         int64_t g = (n_ + 255) / 256;
         if (g > 8192) g = 8192;
         // ring slots k0..k hold q_k0..q_k in order since k0 % kRing == 0
-        hipLaunchKernelGGL(k_lz_accum, dim3((int)g), dim3(256), 0, c->stream,
-                           n_, ne, K, dZ, ring_, n_, V, n_);
+        // (FP32 ring: slots k0 % (2 kRing) + t hold r_k0+t)
+        if (ring32_)
+          hipLaunchKernelGGL((k_lz_accum<float2>), dim3((int)g), dim3(256), 0,
+                             c->stream, n_, ne, K, dZ,
+                             ring32_ + (int64_t)(k0 % (2 * kRing)) * n_, n_, V, n_);
+        else
+          hipLaunchKernelGGL((k_lz_accum<double2>), dim3((int)g), dim3(256), 0,
+                             c->stream, n_, ne, K, dZ, ring_, n_, V, n_);
         NH_HIP(hipGetLastError());
         // the host array hz is reused next chunk: wait for the copy
         NH_HIP(hipStreamSynchronize(c->stream));
