@@ -408,6 +408,8 @@ class LanczosRun {
   ~LanczosRun() {
     release_events();
     for (void *p : allocs_) hipFree(p);
+    if (pin_) hipHostFree(pin_);
+    if (fetch_ev_) hipEventDestroy(fetch_ev_);
   }
   nh_status run(const nh_lanczos_opts *o, nh_eig_result *res);
 
@@ -419,6 +421,7 @@ class LanczosRun {
   }
   nh_status iterate(int k, bool with_reductions);
   nh_status collect_timing();
+  nh_status fetch(int m);
   nh_status analyse(int m, const nh_lanczos_opts *o);
   bool window_done(int w, int req, int rule) const;
 
@@ -437,6 +440,11 @@ class LanczosRun {
   double *alpha_ = nullptr, *beta_ = nullptr, *part_ = nullptr;
   int nparts_ = 0;
   std::vector<double> ha_, hb_;
+  // pinned staging of alpha (first half) and beta, filled up to fetched_
+  // by copies behind fetch_ev_
+  double *pin_ = nullptr;
+  int pin_cap_ = 0, fetched_ = 0;
+  hipEvent_t fetch_ev_ = nullptr;
   std::vector<Converged> conv_;
   std::vector<double> lastRitz_[2];
   std::vector<char> lastConv_[2];
@@ -611,19 +619,27 @@ bool LanczosRun::window_done(int w, int req, int rule) const {
   return true;
 }
 
+// queue the copy of alpha and beta up to step m into the pinned staging
+nh_status LanczosRun::fetch(int m) {
+  if (m > fetched_) {
+    NH_HIP(hipMemcpyAsync(pin_ + fetched_, alpha_ + fetched_,
+                          (m - fetched_) * sizeof(double), hipMemcpyDeviceToHost,
+                          c_->stream));
+    NH_HIP(hipMemcpyAsync(pin_ + pin_cap_ + fetched_, beta_ + fetched_,
+                          (m - fetched_) * sizeof(double), hipMemcpyDeviceToHost,
+                          c_->stream));
+    fetched_ = m;
+  }
+  NH_HIP(hipEventRecord(fetch_ev_, c_->stream));
+  return NH_OK;
+}
+
+// analyse the tridiagonal of the first m steps (fetch(m) queued before)
 nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
-  // copy the new part of the tridiagonal
-  const int have = (int)ha_.size();
-  ha_.resize(m);
-  hb_.resize(m);
-  NH_HIP(hipMemcpyAsync(ha_.data() + have, alpha_ + have,
-                        (m - have) * sizeof(double), hipMemcpyDeviceToHost,
-                        c_->stream));
-  NH_HIP(hipMemcpyAsync(hb_.data() + have, beta_ + have,
-                        (m - have) * sizeof(double), hipMemcpyDeviceToHost,
-                        c_->stream));
-  NH_HIP(hipStreamSynchronize(c_->stream));
-  NH_TRY(collect_timing());
+  NH_CHECK(fetched_ >= m, NH_ERR_STATE, "Lanczos: tridiagonal not fetched");
+  NH_HIP(hipEventSynchronize(fetch_ev_));
+  ha_.assign(pin_, pin_ + m);
+  hb_.assign(pin_ + pin_cap_, pin_ + pin_cap_ + m);
   const double tol = o->conv_tolerance;
   const double resolution = 100.0 * tol;  // lanczos.c:1970
   const double bias = 1e-80;
@@ -712,12 +728,21 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   }
   NH_TRY(upload_start(r0, R32_));
 
-  // pass 1: eigenvalues
-  int m = 0, skip = 0;
+  // pass 1: eigenvalues.  The host analyses the tridiagonal of one check
+  // point while the device runs the steps up to the next one: at a check
+  // point the new alpha, beta are copied behind an event, and they are
+  // analysed at the following check point, after that block of steps has
+  // been queued.  Convergence is thus seen one block late; the steps of
+  // that block are not used (the run ends at the converged check point).
+  pin_cap_ = o->max_iter + 1;
+  NH_HIP(hipHostMalloc((void **)&pin_, 2 * (size_t)pin_cap_ * sizeof(double),
+                       hipHostMallocDefault));
+  NH_HIP(hipEventCreateWithFlags(&fetch_ev_, hipEventDisableTiming));
+  int m = 0, skip = 0, pending = -1;
   bool done = false;
   int matvecs = 0;
   Progress prog;
-  while (!done && m + 1 < o->max_iter) {
+  while (m + 1 < o->max_iter) {
     NH_TRY(iterate(m, true));
     ++m;
     prog.tick("lanczos", m, (int)conv_.size());
@@ -725,12 +750,24 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     ++skip;
     if (o->conv_check_start < m && o->conv_check_skip <= skip) {
       skip = 0;
-      NH_TRY(analyse(m, o));
-      done = window_done(0, o->num_eig_vb, o->stop_rule) &&
-             window_done(1, o->num_eig_cb, o->stop_rule);
+      if (pending > 0) {
+        NH_TRY(analyse(pending, o));
+        done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+               window_done(1, o->num_eig_cb, o->stop_rule);
+        if (done) break;
+      }
+      NH_TRY(fetch(m));
+      pending = m;
     }
   }
-  if (!done) NH_TRY(analyse(m, o));
+  if (done) {
+    m = pending;
+  } else {
+    NH_TRY(fetch(m));
+    NH_TRY(analyse(m, o));
+    done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+           window_done(1, o->num_eig_cb, o->stop_rule);
+  }
   if (res->tridiag_alpha && res->tridiag_beta) {
     const int cnt = std::min(m, res->tridiag_cap);
     std::copy(ha_.begin(), ha_.begin() + cnt, res->tridiag_alpha);
