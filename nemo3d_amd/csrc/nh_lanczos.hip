@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -396,6 +397,23 @@ void tri_eigvec(const std::vector<double> &d, const std::vector<double> &e,
   }
 }
 
+// f(j) for j < count on up to 8 host threads (independent j)
+template <typename F>
+void host_parallel_for(int count, F f) {
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int nt = std::min({count, hw, 8});
+  if (nt <= 1) {
+    for (int j = 0; j < count; ++j) f(j);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&f, t, nt, count] {
+      for (int j = t; j < count; j += nt) f(j);
+    });
+  for (auto &th : pool) th.join();
+}
+
 struct Converged {
   double value;
   int iter;
@@ -643,7 +661,6 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
   const double tol = o->conv_tolerance;
   const double resolution = 100.0 * tol;  // lanczos.c:1970
   const double bias = 1e-80;
-  std::vector<double> x;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};
   const double *win[2] = {o->erange_vb, o->erange_cb};
   for (int w = 0; w < 2; ++w) {
@@ -657,21 +674,35 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
     int k0 = klo, k1 = khi;
     if (w == 0) k0 = std::max(klo, khi - want);
     else k1 = std::min(khi, klo + want);
-    std::vector<double> ev;
-    for (int k = k0; k < k1; ++k)
-      ev.push_back(bisect(ha_, hb_, m, k, lo, hi, 1e-13));
-    for (size_t i = 0; i < ev.size(); ++i) {
+    // bisection of the candidates and the trailing components of their
+    // tridiagonal eigenvectors, each independent: spread over host threads
+    std::vector<double> ev(std::max(0, k1 - k0));
+    host_parallel_for((int)ev.size(), [&](int j) {
+      ev[j] = bisect(ha_, hb_, m, k0 + j, lo, hi, 1e-13);
+    });
+    auto duplicate = [&](size_t i) {
       // copies of one eigenvalue (ghosts, degeneracies) count once
-      if (i > 0 && std::fabs(ev[i - 1] - ev[i]) <
-                       std::fabs(bias + ev[i]) * resolution)
-        continue;
-      bool known = false;
+      return i > 0 && std::fabs(ev[i - 1] - ev[i]) < std::fabs(bias + ev[i]) * resolution;
+    };
+    auto known_value = [&](double v) {
       for (const auto &cv : conv_)
-        if (std::fabs(cv.value - ev[i]) < resolution) known = true;
+        if (std::fabs(cv.value - v) < resolution) return true;
+      return false;
+    };
+    std::vector<double> tail((size_t)2 * ev.size(), 1.0);
+    host_parallel_for((int)ev.size(), [&](int j) {
+      if (duplicate(j) || known_value(ev[j])) return;
+      std::vector<double> xj;
+      tri_eigvec(ha_, hb_, m, ev[j], xj);
+      tail[2 * j] = xj[m - 1];
+      tail[2 * j + 1] = m >= 2 ? xj[m - 2] : 0.0;
+    });
+    for (size_t i = 0; i < ev.size(); ++i) {
+      if (duplicate(i)) continue;
+      const bool known = known_value(ev[i]);
       bool ok = known;
       if (!known) {
-        tri_eigvec(ha_, hb_, m, ev[i], x);
-        ok = std::fabs(x[m - 1]) <= tol && (m < 2 || std::fabs(x[m - 2]) <= tol);
+        ok = std::fabs(tail[2 * i]) <= tol && std::fabs(tail[2 * i + 1]) <= tol;
         if (ok) conv_.push_back({ev[i], m, w});
       }
       lastRitz_[w].push_back(ev[i]);
