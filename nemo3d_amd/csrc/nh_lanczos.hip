@@ -12,8 +12,8 @@
 // tridiagonal eigenvector, find_converged_eigval Method 2,
 // lanczos.c:764-996).  Eigenvectors come from a second pass over the same
 // recurrence (eigenvectorsLanczos, lanczos.c:1364-1689): the Lanczos
-// vectors land in a ring of kRing slots and every kRing iterations one
-// kernel adds Ring * Z to the eigenvector block.  The second pass reuses
+// vectors land in a ring of K slots (64, or 32 when memory is short) and
+// every K iterations one kernel adds Ring * Z to the eigenvector block.  The second pass reuses
 // alpha and beta, so with the SK layout its H*v kernel also applies the
 // three-term step (no separate update kernel, no reductions).
 #include <algorithm>
@@ -30,7 +30,7 @@
 namespace nh {
 namespace {
 
-constexpr int kRing = 32;
+constexpr int kRingMax = 64;
 constexpr int kMaxVec = 64;
 
 __device__ __forceinline__ double wg_sum320(double v) {
@@ -278,38 +278,52 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_scalar(
   if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(a) : a;
 }
 
-// V[e] += sum_k Z[e][k] Ring[k] for e < ne, k < K (ring of FP64 q or of
-// the FP32 copies of r).
+// V[e] += sum_k Z[e][k] Ring[k] for e < ne, k < K <= kRingMax (ring of
+// FP64 q or of the FP32 copies of r).  Eight eigenvectors at a time stay in
+// registers while the ring streams past in groups of eight slots.
 template <typename T>
 __global__ __launch_bounds__(256) void k_lz_accum(
     int64_t n, int ne, int K, const double *__restrict__ Z,
     const T *__restrict__ ring, int64_t ring_ld, double2 *__restrict__ V,
     int64_t v_ld) {
-  __shared__ double zs[kMaxVec * kRing];
-  for (int t = threadIdx.x; t < ne * K; t += blockDim.x) zs[t] = Z[t];
-  __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double2 q[kRing];
-#pragma unroll
-    for (int k = 0; k < kRing; ++k) {
-      q[k] = make_double2(0.0, 0.0);
-      if (k < K) {
-        const T v = ring[(int64_t)k * ring_ld + i];
-        q[k] = make_double2(v.x, v.y);
-      }
+  __shared__ double zs[8 * kRingMax];
+  const int K8 = (K + 7) / 8 * 8;
+  for (int e0 = 0; e0 < ne; e0 += 8) {
+    const int nb = min(8, ne - e0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 8 * K8; t += blockDim.x) {
+      const int e = t / K8, k = t % K8;
+      zs[t] = e < nb && k < K ? Z[(e0 + e) * K + k] : 0.0;
     }
-    for (int e = 0; e < ne; ++e) {
-      double2 v = V[(int64_t)e * v_ld + i];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      double2 v[8];
 #pragma unroll
-      for (int k = 0; k < kRing; ++k) {
-        const double z = zs[e * K + (k < K ? k : 0)];
-        if (k < K) {
-          v.x += z * q[k].x;
-          v.y += z * q[k].y;
+      for (int e = 0; e < 8; ++e)
+        v[e] = e < nb ? V[(int64_t)(e0 + e) * v_ld + i] : make_double2(0.0, 0.0);
+      for (int k0 = 0; k0 < K8; k0 += 8) {
+        double2 q[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          q[j] = make_double2(0.0, 0.0);
+          if (k0 + j < K) {
+            const T r = ring[(int64_t)(k0 + j) * ring_ld + i];
+            q[j] = make_double2(r.x, r.y);
+          }
         }
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const double z = zs[e * K8 + k0 + j];
+            v[e].x += z * q[j].x;
+            v[e].y += z * q[j].y;
+          }
       }
-      V[(int64_t)e * v_ld + i] = v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e < nb) V[(int64_t)(e0 + e) * v_ld + i] = v[e];
     }
   }
 }
@@ -449,8 +463,10 @@ class LanczosRun {
   double2 *R_ = nullptr, *Y_ = nullptr, *Rm1_ = nullptr, *ring_ = nullptr;
   // FP32 copies of r for the FP32 SK products (null otherwise)
   float2 *R32_ = nullptr, *Y32_ = nullptr;
-  // eigenvector pass with FP32 products: 2 kRing FP32 copies of r, slot
-  // k % (2 kRing) holding r_k; the gather of step k reads slot k and writes
+  // ring length of the eigenvector pass (<= kRingMax)
+  int ring_k_ = kRingMax;
+  // eigenvector pass with FP32 products: 2 K FP32 copies of r, slot
+  // k % (2 K) holding r_k; the gather of step k reads slot k and writes
   // r_{k+1} to slot k + 1, and the accumulation reads them as its ring
   // (1 / beta_{k-1} folded into the coefficients)
   float2 *ring32_ = nullptr;
@@ -525,13 +541,13 @@ nh_status LanczosRun::iterate(int k, bool red) {
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   // q_k is stored (ring slot) only in the eigenvector pass, which
   // accumulates from it; the update forms q_k, q_{k-1} from r_k, r_{k-1}
-  double2 *Q = red || ring32_ ? nullptr : ring_ + (int64_t)(k % kRing) * n_;
+  double2 *Q = red || ring32_ ? nullptr : ring_ + (int64_t)(k % ring_k_) * n_;
   // FP32 copies: gathered (r_k) and written (r_{k+1})
   const float2 *g32 = R32_;
   float2 *y32 = Y32_;
   if (!red && ring32_) {
-    g32 = ring32_ + (int64_t)(k % (2 * kRing)) * n_;
-    y32 = ring32_ + (int64_t)((k + 1) % (2 * kRing)) * n_;
+    g32 = ring32_ + (int64_t)(k % (2 * ring_k_)) * n_;
+    y32 = ring32_ + (int64_t)((k + 1) % (2 * ring_k_)) * n_;
   }
   const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
   const double *bprev2 = k > 1 ? beta_ + (k - 2) : nullptr;
@@ -857,19 +873,28 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     double2 *V = nullptr;
     double *dZ = nullptr;
     NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
-    NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRing * sizeof(double)));
+    NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRingMax * sizeof(double)));
     NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
+    // ring: 64 slots when they take at most a quarter of the free memory
+    // (each flush then reads and writes the eigenvectors once per 64 steps)
+    {
+      size_t fr = 0, tot = 0;
+      NH_HIP(hipMemGetInfo(&fr, &tot));
+      const size_t slot = R32_ ? 2 * n_ * sizeof(float2) : n_ * sizeof(double2);
+      ring_k_ = (size_t)kRingMax * slot <= fr / 4 ? kRingMax : kRingMax / 2;
+    }
     if (R32_)
-      NH_TRY(alloc((void **)&ring32_, (size_t)2 * kRing * n_ * sizeof(float2)));
+      NH_TRY(alloc((void **)&ring32_, (size_t)2 * ring_k_ * n_ * sizeof(float2)));
     else
-      NH_TRY(alloc((void **)&ring_, (size_t)kRing * n_ * sizeof(double2)));
+      NH_TRY(alloc((void **)&ring_, (size_t)ring_k_ * n_ * sizeof(double2)));
     NH_TRY(upload_start(r0, ring32_));
-    std::vector<double> hz((size_t)kMaxVec * kRing);
+    std::vector<double> hz((size_t)kMaxVec * kRingMax);
+    const int rk = ring_k_;
     for (int k = 0; k < mmax; ++k) {
       NH_TRY(iterate(k, false));
       ++matvecs;
-      if ((k + 1) % kRing == 0 || k + 1 == mmax) {
-        const int k0 = k + 1 - ((k % kRing) + 1);
+      if ((k + 1) % rk == 0 || k + 1 == mmax) {
+        const int k0 = k + 1 - ((k % rk) + 1);
         const int K = k + 1 - k0;
         for (int e = 0; e < ne; ++e)
           for (int t = 0; t < K; ++t) {
@@ -882,12 +907,12 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
                               hipMemcpyHostToDevice, c->stream));
         int64_t g = (n_ + 255) / 256;
         if (g > 8192) g = 8192;
-        // ring slots k0..k hold q_k0..q_k in order since k0 % kRing == 0
-        // (FP32 ring: slots k0 % (2 kRing) + t hold r_k0+t)
+        // ring slots k0..k hold q_k0..q_k in order since k0 % K == 0
+        // (FP32 ring: slots k0 % (2 K) + t hold r_k0+t)
         if (ring32_)
           hipLaunchKernelGGL((k_lz_accum<float2>), dim3((int)g), dim3(256), 0,
                              c->stream, n_, ne, K, dZ,
-                             ring32_ + (int64_t)(k0 % (2 * kRing)) * n_, n_, V, n_);
+                             ring32_ + (int64_t)(k0 % (2 * rk)) * n_, n_, V, n_);
         else
           hipLaunchKernelGGL((k_lz_accum<double2>), dim3((int)g), dim3(256), 0,
                              c->stream, n_, ne, K, dZ, ring_, n_, V, n_);
