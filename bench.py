@@ -103,15 +103,20 @@ def log(msg):
     print("# " + msg, file=sys.stderr, flush=True)
 
 
+# convergence check (tridiagonal analysis) every CHECK_SKIP steps; the host
+# analysis overlaps the next block of device steps
+CHECK_SKIP = 50
+
+
 def plain_solve(H, max_iter):
     return eigensolve_lanczos(H, num_vb=4, num_cb=4, max_iter=max_iter, tol=1e-8,
-                              start_iter=200, skip=100, stop_rule="window",
+                              start_iter=200, skip=CHECK_SKIP, stop_rule="window",
                               **WINDOWS)
 
 
 def mixed_solve(H, start, max_iter):
     return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, hnorm=HNORM,
-                            max_iter=max_iter, start_iter=200, skip=100,
+                            max_iter=max_iter, start_iter=200, skip=CHECK_SKIP,
                             start_vector=start, stop_rule="window",
                             refine_depth=12, refine_rounds=4, **WINDOWS)
 
