@@ -202,14 +202,22 @@ __device__ __forceinline__ void sk_load_x(const float2 *__restrict__ x,
 template <int NSP, typename XT>
 struct SkQuadStage {
   static constexpr int kChunks = NSP * 10 * (int)sizeof(XT) / 16;  // 20/10/5
-  // Lane t reads its item at stride kStride chunks: with a multiple of 4
-  // chunks the chunk index is XOR-swizzled by (t >> 2) & 3 (16 lanes then
-  // cover all 64 banks; 80 KB for 256 lanes in FP64, two workgroups per
-  // CU), otherwise one chunk of padding.
+  // Lane t keeps its item in chunks [t kChunks, (t + 1) kChunks), no
+  // padding.  A multiple of 4 chunks: chunk index XOR-swizzled by
+  // (t >> 2) & 3 (16 lanes then cover all 64 banks); 2 mod 4: chunk index
+  // rotated by one for lanes with bit 2 ^ bit 3 set, so the reads of 8 or
+  // 16 consecutive lanes hit distinct banks (the quad writes stay 2-way);
+  // odd: already conflict-free.
   static constexpr bool kSwizzle = kChunks % 4 == 0;
-  static constexpr int kStride = kSwizzle ? kChunks : kChunks + 1;
+  static constexpr bool kRotate = kChunks % 4 == 2;
+  static constexpr int kStride = kChunks;
   __device__ static int pos(int t, int c) {
-    return t * kStride + (kSwizzle ? (c ^ ((t >> 2) & 3)) : c);
+    if constexpr (kSwizzle) return t * kStride + (c ^ ((t >> 2) & 3));
+    if constexpr (kRotate) {
+      const int r = c + (((t >> 2) ^ (t >> 3)) & 1);
+      return t * kStride + (r >= kChunks ? r - kChunks : r);
+    }
+    return t * kStride + c;
   }
   uint4 buf[kSkThreads * kStride];
 };

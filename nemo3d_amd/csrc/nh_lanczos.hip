@@ -133,7 +133,7 @@ __device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
 }
 
 template <int NSP, typename R, bool QUAD, bool FUSE>
-__global__ __launch_bounds__(kSkThreads, 2) void k_lz_hv_sk(
+__global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, int ppx,
     const double *__restrict__ onsite, const R *__restrict__ sk,
