@@ -27,6 +27,12 @@ Forms:
 `value` is the time of the mixed solve; the plain time, the speed-up and
 every eigenvalue's difference to the plain FP64 result are in `config`.
 
+--workload pyramid4m runs the large single-GPU shape instead: a truncated
+pyramid In0.7Ga0.3As dot (base 30 nm, top 12 nm, height 8 nm, seeded
+alloy, 1-ML wetting layer) in a 50 x 50 x 40 nm GaAs box, 4,336,640 atoms,
+N = 86.7M, with the Lanczos vector memory in `config` (no CPU baseline: the
+oracle's sample alone would take minutes).
+
 Prints one JSON line (the repository's bench contract).
 """
 import argparse
@@ -46,15 +52,53 @@ from nemo3d_amd.hamiltonian import Hamiltonian  # noqa: E402
 from nemo3d_amd.solvers import eigensolve_lanczos, eigensolve_mixed  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-HNORM = 40.0   # spectral radius of this H (Ritz extremes -12.4 / 39.4 eV)
-WINDOWS = dict(erange_vb=(0.60, 0.70), erange_cb=(1.45, 1.52))
 WEIGHTS = os.path.join(HERE, "nemo3d_amd", "data", "warmstart.npz")
 
 
-def build_dot():
-    M = warmstart.dot_model((30.0, 30.0, 20.0), 20.0, 5.0, 1.0, 0)
+def build_dot786k():
+    return warmstart.dot_model((30.0, 30.0, 20.0), 20.0, 5.0, 1.0, 0)
+
+
+def build_pyramid4m():
+    return warmstart.dot_model((50.0, 50.0, 40.0), 30.0, 8.0, 0.7, 4,
+                               shape="pyramid", top_nm=12.0)
+
+
+# hnorm: spectral radius of H (Lanczos Ritz extremes -12.4 / 39.4 eV and
+# -12.8 / 38.4 eV); windows: 4 VB / 4 CB levels at the gap edges (found by
+# a 3,000-step Lanczos scan of each structure)
+WORKLOADS = {
+    "dot786k": dict(
+        build=build_dot786k, hnorm=40.0,
+        windows=dict(erange_vb=(0.60, 0.70), erange_cb=(1.45, 1.52)),
+        text="InAs dome dot 20x5 nm + 1 ML WL in 30x30x20 nm GaAs"),
+    "pyramid4m": dict(
+        build=build_pyramid4m, hnorm=40.0,
+        windows=dict(erange_vb=(0.50, 0.56), erange_cb=(1.43, 1.48)),
+        text="In0.7Ga0.3As truncated pyramid dot 30/12x8 nm (seeded alloy) "
+             "+ 1 ML WL in 50x50x40 nm GaAs"),
+}
+HNORM = WORKLOADS["dot786k"]["hnorm"]
+WINDOWS = WORKLOADS["dot786k"]["windows"]
+
+
+def build_dot(workload="dot786k"):
+    M = WORKLOADS[workload]["build"]()
     info = vff.relax(M)
     return M, info
+
+
+def lanczos_memory(n, k=8):
+    """Device bytes of the Lanczos vectors at N = n: the eigenvalue pass
+    keeps r_k, r_(k-1) and y in FP64 (+ two FP32 copies of r with FP32
+    products); the eigenvector pass adds the ring (64 FP64 slots, or 2 x 64
+    FP32 copies; 32 when memory is short) and the k FP64 eigenvectors."""
+    v64, v32 = n * 16, n * 8
+    return {"eigenvalue_pass_fp64_GB": 3 * v64 / 1e9,
+            "eigenvalue_pass_fp32_products_GB": (3 * v64 + 2 * v32) / 1e9,
+            "eigenvector_pass_fp64_GB": (3 * v64 + 64 * v64 + k * v64) / 1e9,
+            "eigenvector_pass_fp32_products_GB":
+                (3 * v64 + 2 * v32 + 128 * v32 + k * v64) / 1e9}
 
 
 def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False):
@@ -108,17 +152,17 @@ def log(msg):
 CHECK_SKIP = 50
 
 
-def plain_solve(H, max_iter):
+def plain_solve(H, max_iter, windows=WINDOWS):
     return eigensolve_lanczos(H, num_vb=4, num_cb=4, max_iter=max_iter, tol=1e-8,
                               start_iter=200, skip=CHECK_SKIP, stop_rule="window",
-                              **WINDOWS)
+                              **windows)
 
 
-def mixed_solve(H, start, max_iter):
-    return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, hnorm=HNORM,
+def mixed_solve(H, start, max_iter, windows=WINDOWS, hnorm=HNORM):
+    return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, hnorm=hnorm,
                             max_iter=max_iter, start_iter=200, skip=CHECK_SKIP,
                             start_vector=start, stop_rule="window",
-                            refine_depth=12, refine_rounds=4, **WINDOWS)
+                            refine_depth=12, refine_rounds=4, **windows)
 
 
 def main():
@@ -130,14 +174,17 @@ def main():
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iter", type=int, default=30000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dot786k")
     args = ap.parse_args()
+    wl = WORKLOADS[args.workload]
+    hnorm, windows = wl["hnorm"], wl["windows"]
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.time()
-    M, vinfo = build_dot()
+    M, vinfo = build_dot(args.workload)
     H = Hamiltonian(M, layout=native.LAYOUT_SK, precision=native.PREC_FP64)
     H.set_precision(native.PREC_FP32)  # FP32 copies made outside the timing
     H.set_precision(native.PREC_FP64)
@@ -155,7 +202,7 @@ def main():
 
     def run_plain():
         t = time.time()
-        r = plain_solve(H, args.max_iter)
+        r = plain_solve(H, args.max_iter, windows)
         return time.time() - t, r
 
     def run_mixed():
@@ -166,7 +213,7 @@ def main():
             start = warmstart.start_vector_device(H, blk)
             del blk
         t_cnn = time.time() - t
-        r = mixed_solve(H, start, args.max_iter)
+        r = mixed_solve(H, start, args.max_iter, windows, hnorm)
         r["cnn_s"] = t_cnn
         return time.time() - t, r
 
@@ -194,14 +241,14 @@ def main():
 
     headline = mixed if mixed is not None else plain
     sec = float(np.mean(tm if mixed is not None else tp))
-    rel = (np.asarray(headline["residuals"]) / HNORM).tolist()
+    rel = (np.asarray(headline["residuals"]) / hnorm).tolist()
     cfg = {
-        "workload": "InAs dome dot 20x5 nm + 1 ML WL in 30x30x20 nm GaAs, "
-                    "sp3d5s*+SO, %d atoms, N=%d, 4 VB + 4 CB levels @1e-8 "
-                    "rel. residual (||H|| = %g eV)" % (M.n_atoms, M.n, HNORM),
+        "workload": "%s, sp3d5s*+SO, %d atoms, N=%d, 4 VB + 4 CB levels @1e-8 "
+                    "rel. residual (||H|| = %g eV)" % (wl["text"], M.n_atoms,
+                                                       M.n, hnorm),
         "positions": "VFF-relaxed (device Keating, %d CG iterations)"
                      % vinfo.get("iterations", 0),
-        "windows_eV": WINDOWS,
+        "windows_eV": windows,
         "form": "mixed+learned" if (mixed is not None and net is not None)
                 else ("mixed (random start)" if mixed is not None else "plain"),
         "eigenvalues_eV": np.round(headline["eigenvalues"], 9).tolist(),
@@ -257,7 +304,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(tr_key),
+            # measured (PMC) on the default workload only
+            "traffic": traffic_from_profiles(tr_key) if args.workload == "dot786k" else None,
             "bytes_per_launch": nbytes,
             "launch_ms": kms,
         },
@@ -269,9 +317,12 @@ def main():
             "achieved": pb / (plain["hv_ms"] * 1e-3) / 1e9 if plain["hv_ms"] > 0 else 0.0,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": pb,
             "launch_ms": plain["hv_ms"],
-            "traffic": traffic_from_profiles("k_lz_hv_sk")}
+            "traffic": traffic_from_profiles("k_lz_hv_sk")
+            if args.workload == "dot786k" else None}
         line["roofline_plain"]["frac"] = line["roofline_plain"]["achieved"] / HBM_PEAK_GBS
-    if not args.no_cpu_baseline and plain is not None:
+    if args.workload != "dot786k":
+        cfg["lanczos_vector_memory"] = lanczos_memory(M.n)
+    if not args.no_cpu_baseline and plain is not None and args.workload == "dot786k":
         line["cpu_baseline"] = cpu_baseline(M, plain["matvecs"])
     print(json.dumps(line), flush=True)
 

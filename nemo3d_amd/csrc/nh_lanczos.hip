@@ -893,6 +893,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     for (int k = 0; k < mmax; ++k) {
       NH_TRY(iterate(k, false));
       ++matvecs;
+      prog.tick("lanczos eigenvectors", k + 1, mmax);
       if ((k + 1) % rk == 0 || k + 1 == mmax) {
         const int k0 = k + 1 - ((k % rk) + 1);
         const int K = k + 1 - k0;
