@@ -158,8 +158,16 @@ def plain_solve(H, max_iter, windows=WINDOWS):
                               **windows)
 
 
+# the FP32 stage stops at 3e-8 (its vectors cannot go below the FP32
+# rounding of H·v, ~1e-7 ||H||); one FP64 refinement round takes them to
+# 1e-8.  Measured on the 786k dot: 1e-8 6.82 s, 3e-8 6.65 s, 1e-7 6.82 s
+# (two rounds), 3e-7 not converged after four rounds (neighbouring levels
+# outside the refinement's block).
+TOL32 = 3e-8
+
+
 def mixed_solve(H, start, max_iter, windows=WINDOWS, hnorm=HNORM):
-    return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, hnorm=hnorm,
+    return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, tol32=TOL32, hnorm=hnorm,
                             max_iter=max_iter, start_iter=200, skip=CHECK_SKIP,
                             start_vector=start, stop_rule="window",
                             refine_depth=12, refine_rounds=4, **windows)
