@@ -272,8 +272,16 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_update(
 __global__ __launch_bounds__(kRedThreads) void k_lz_scalar(
     int nparts, int stride, const double *__restrict__ part,
     double *__restrict__ out, int take_sqrt) {
-  double a = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < nparts; k += kRedThreads) a += part[k * stride];
+  // eight independent partial sums per thread (all loads of a trip in
+  // flight together), combined in a fixed order
+  double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int k = threadIdx.x;
+  for (; k + 7 * kRedThreads < nparts; k += 8 * kRedThreads) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] += part[(k + u * kRedThreads) * stride];
+  }
+  for (int u = 0; k < nparts; k += kRedThreads, ++u) p[u & 7] += part[k * stride];
+  double a = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7])), b = 0.0;
   block_sum2(a, b);
   if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(a) : a;
 }
