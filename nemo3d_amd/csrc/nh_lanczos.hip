@@ -336,6 +336,62 @@ __global__ __launch_bounds__(256) void k_lz_accum(
   }
 }
 
+// The FP32 ring (copies of r) accumulated two rows per thread, so each
+// ring load is 16 bytes; n even.
+__global__ __launch_bounds__(256) void k_lz_accum32(
+    int64_t n, int ne, int K, const double *__restrict__ Z,
+    const float2 *__restrict__ ring, int64_t ring_ld, double2 *__restrict__ V,
+    int64_t v_ld) {
+  __shared__ double zs[8 * kRingMax];
+  const int K8 = (K + 7) / 8 * 8;
+  const int64_t n2 = n / 2;
+  const float4 *r4 = reinterpret_cast<const float4 *>(ring);
+  const int64_t ld4 = ring_ld / 2;
+  for (int e0 = 0; e0 < ne; e0 += 8) {
+    const int nb = min(8, ne - e0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 8 * K8; t += blockDim.x) {
+      const int e = t / K8, k = t % K8;
+      zs[t] = e < nb && k < K ? Z[(e0 + e) * K + k] : 0.0;
+    }
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      double2 v0[8], v1[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        v0[e] = v1[e] = make_double2(0.0, 0.0);
+        if (e < nb) {
+          v0[e] = V[(int64_t)(e0 + e) * v_ld + 2 * i];
+          v1[e] = V[(int64_t)(e0 + e) * v_ld + 2 * i + 1];
+        }
+      }
+      for (int k0 = 0; k0 < K8; k0 += 8) {
+        float4 q[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          q[j] = k0 + j < K ? r4[(int64_t)(k0 + j) * ld4 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const double z = zs[e * K8 + k0 + j];
+            v0[e].x += z * (double)q[j].x;
+            v0[e].y += z * (double)q[j].y;
+            v1[e].x += z * (double)q[j].z;
+            v1[e].y += z * (double)q[j].w;
+          }
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (e < nb) {
+          V[(int64_t)(e0 + e) * v_ld + 2 * i] = v0[e];
+          V[(int64_t)(e0 + e) * v_ld + 2 * i + 1] = v1[e];
+        }
+    }
+  }
+}
+
 __global__ void k_scale(int64_t n, double2 *__restrict__ x, double s) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x) {
@@ -891,8 +947,10 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
       const size_t slot = R32_ ? 2 * n_ * sizeof(float2) : n_ * sizeof(double2);
       ring_k_ = (size_t)kRingMax * slot <= fr / 4 ? kRingMax : kRingMax / 2;
     }
-    if (R32_)
+    if (R32_) {
+      NH_CHECK(n_ % 2 == 0, NH_ERR_STATE, "Lanczos: FP32 ring needs an even N");
       NH_TRY(alloc((void **)&ring32_, (size_t)2 * ring_k_ * n_ * sizeof(float2)));
+    }
     else
       NH_TRY(alloc((void **)&ring_, (size_t)ring_k_ * n_ * sizeof(double2)));
     NH_TRY(upload_start(r0, ring32_));
@@ -919,7 +977,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
         // ring slots k0..k hold q_k0..q_k in order since k0 % K == 0
         // (FP32 ring: slots k0 % (2 K) + t hold r_k0+t)
         if (ring32_)
-          hipLaunchKernelGGL((k_lz_accum<float2>), dim3((int)g), dim3(256), 0,
+          hipLaunchKernelGGL(k_lz_accum32, dim3((int)g), dim3(256), 0,
                              c->stream, n_, ne, K, dZ,
                              ring32_ + (int64_t)(k0 % (2 * rk)) * n_, n_, V, n_);
         else
