@@ -196,6 +196,46 @@ def test_mixed_precision_matches_dense():
     assert H.precision == native.PREC_FP64
 
 
+def test_fp32_lanczos_vectors_at_fp32_level():
+    """Lanczos with FP32 products: the eigenvector pass accumulates the FP32
+    copies of r (its ring) with 1/beta folded into the coefficients; the
+    vectors' FP64 residuals and eigenvalues sit at the FP32 rounding of H·v,
+    far below the gap to the next level, and the FP64 run stays exact."""
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=24)
+    ref = oracle.assemble(M.arrays())
+    wd, vd = np.linalg.eigh(ref.dense())
+    w = distinct(wd)
+    wv, wc = gap_windows(w)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    out = {}
+    for prec in (native.PREC_FP32, native.PREC_FP64):
+        H.set_precision(prec)
+        out[prec] = eigensolve_lanczos(H, num_vb=2, num_cb=2, erange_vb=wv,
+                                       erange_cb=wc, max_iter=6000, tol=1e-7,
+                                       start_iter=100, skip=25,
+                                       return_vectors=True, stop_rule="nearest")
+    H.set_precision(native.PREC_FP64)
+    # bars (relative to ||H|| = 40 eV): the FP64 run to its Lanczos
+    # tolerance; the FP32 run to the FP32 rounding of H·v
+    # (measured: FP32 2.3-2.8e-8, eigenvalues within 2e-7 eV of FP64)
+    for prec, bar_e, bar_r in ((native.PREC_FP32, 1e-6, 2e-7), (native.PREC_FP64, 1e-9, 1e-7)):
+        r = out[prec]
+        assert r["converged"]
+        for e, v in zip(r["eigenvalues"], r["vectors"]):
+            v = v / np.linalg.norm(v)
+            assert np.abs(wd - e).min() < bar_e, (prec, e)
+            res = np.linalg.norm(ref.hv(v) - e * v)
+            print("precision", prec, "E", e, "residual / ||H||", res / 40.0, flush=True)
+            assert res < bar_r * 40.0, (prec, e, res)
+            # the vector lies in the eigenspace of its level (Kramers pair)
+            sel = np.abs(wd - e) < 1e-6
+            P = vd[:, sel]
+            assert np.linalg.norm(P.conj().T @ v) ** 2 > 1 - 1e-6, (prec, e)
+    print("fp32 vs fp64 eigenvalues",
+          np.sort(out[native.PREC_FP32]["eigenvalues"]) - np.sort(out[native.PREC_FP64]["eigenvalues"]),
+          flush=True)
+
+
 def test_refine_eigenpairs_from_perturbed_vectors():
     """FP64 refinement of dense eigenvectors perturbed by noise: residuals
     fall below the tolerance and the values reach the dense ones."""
