@@ -743,6 +743,19 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
   const double bias = 1e-80;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};
   const double *win[2] = {o->erange_vb, o->erange_cb};
+  // nearest-the-gap rule: two converged copies of one level differ by up to
+  // about 2 tol ||T|| (each Ritz value within beta tol of it), which the
+  // reference's relative resolution misses for levels near 0 eV; that rule
+  // merges copies within this bound as well (the reference rule keeps the
+  // reference's check)
+  double dup_abs = 0.0;
+  if (o->stop_rule == 1) {
+    double tn = 0.0;
+    for (int i = 0; i < m; ++i)
+      tn = std::max(tn, std::fabs(ha_[i]) + (i > 0 ? std::fabs(hb_[i - 1]) : 0.0) +
+                            (i + 1 < m ? std::fabs(hb_[i]) : 0.0));
+    dup_abs = 2.0 * tol * tn;
+  }
   for (int w = 0; w < 2; ++w) {
     lastRitz_[w].clear();
     lastConv_[w].clear();
@@ -762,7 +775,8 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
     });
     auto duplicate = [&](size_t i) {
       // copies of one eigenvalue (ghosts, degeneracies) count once
-      return i > 0 && std::fabs(ev[i - 1] - ev[i]) < std::fabs(bias + ev[i]) * resolution;
+      return i > 0 && std::fabs(ev[i - 1] - ev[i]) <
+                          std::max(std::fabs(bias + ev[i]) * resolution, dup_abs);
     };
     auto known_value = [&](double v) {
       for (const auto &cv : conv_)
