@@ -23,12 +23,12 @@
 namespace nh {
 namespace {
 
-constexpr int kTile = 8;  // block padding: columns per Gram tile
 
 class Refine {
  public:
   Refine(nh_context *c, int k, int depth, double tol, double hnorm)
-      : c_(c), k_(k), d_(depth), kp_((k + kTile - 1) / kTile * kTile),
+      : c_(c), k_(k), d_(depth), kp_(k),  // blocks unpadded: the Gram and
+                                             // update kernels take any width
         tol_(tol), hnorm_(hnorm), ops_(c, kp_, (depth + 1) * kp_) {
     n_ = c->n;
   }

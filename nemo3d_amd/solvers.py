@@ -208,22 +208,25 @@ def eigensolve_pctracemin(H, num_eig=4, want_range=(0.6, 0.7),
 
 
 def refine_eigenpairs(H, X, eigenvalues, tol=1e-8, hnorm=40.0, depth=6,
-                      max_rounds=4):
+                      max_rounds=4, row0=0):
     """FP64 refinement of approximate eigenpairs (nh_refine_eigenpairs):
     block Krylov [X, H X, ..., H^depth X] and refined Ritz vectors per round
     until every ||H x - e x|| <= tol * hnorm.  X: (k, N) host array (a
-    refined copy is returned) or native.DeviceArray of shape (>= k, N)
-    (refined in place, k = len(eigenvalues))."""
+    refined copy is returned) or native.DeviceArray of shape (>= row0 + k,
+    N) whose rows row0 .. row0 + k - 1 are refined in place (k =
+    len(eigenvalues))."""
     lib = native.load()
     ev = np.array(eigenvalues, dtype=np.float64, copy=True)
     k = ev.shape[0]
     if isinstance(X, native.DeviceArray):
-        assert X.dtype == np.complex128 and X.shape[0] >= k and X.shape[1] == H.n
+        assert X.dtype == np.complex128 and X.shape[0] >= row0 + k and X.shape[1] == H.n
         Xc, kind = X, native.PTR_DEVICE
+        xp = ctypes.c_void_p(X.ptr.value + row0 * H.n * 16)
     else:
         Xc = np.array(X, dtype=np.complex128, order="C", copy=True)
-        assert Xc.shape == (k, H.n)
+        assert Xc.shape == (k, H.n) and row0 == 0
         kind = native.PTR_HOST
+        xp = native.ptr(Xc)
     assert 1 <= k <= 16
     rs = np.zeros(k)
     o = native.NHRefineOpts()
@@ -231,8 +234,7 @@ def refine_eigenpairs(H, X, eigenvalues, tol=1e-8, hnorm=40.0, depth=6,
     o.tolerance, o.hnorm = float(tol), float(hnorm)
     rounds, mv = ctypes.c_int32(), ctypes.c_int32()
     t0 = time.perf_counter()
-    st = lib.nh_refine_eigenpairs(H.ctx, ctypes.byref(o), k, native.ptr(Xc),
-                                  kind, native.ptr(ev), native.ptr(rs),
+    st = lib.nh_refine_eigenpairs(H.ctx, ctypes.byref(o), k, xp, kind, native.ptr(ev), native.ptr(rs),
                                   ctypes.byref(rounds), ctypes.byref(mv))
     if st not in (native.NH_OK, 5):
         native.check(st)
@@ -246,13 +248,15 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                      erange_cb=(1.4, 2.0), tol=1e-8, tol32=None, hnorm=40.0,
                      max_iter=30000, start_iter=200, skip=50, start_vector=None,
                      stop_rule="window", refine_depth=12, refine_rounds=4,
-                     return_vectors=False):
+                     return_vectors=False, refine_split=True):
     """Mixed-precision Lanczos: the Lanczos run and its eigenvector pass
     with FP32 H·v (FP32 values and products inside each block, FP64 sums
     over the bonds) while the recurrence, dots and the tridiagonal stay
     FP64; then FP64 residual refinement (refine_eigenpairs) of its Ritz
     pairs to ||H x - e x|| <= tol * hnorm.  tol32: convergence tolerance of
-    the FP32 stage (default tol).  start_vector: optional start, e.g. from
+    the FP32 stage (default tol).  refine_split: refine the valence and the
+    conduction pairs as two blocks (default) instead of one.  start_vector:
+    optional start, e.g. from
     the learned start block."""
     from . import native as nt
     prec0 = H.precision
@@ -270,14 +274,26 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
             capacity=cap, stop_rule=stop_rule, vectors_out=dv)
         H.set_precision(nt.PREC_FP64)
         n = len(r1["eigenvalues"])
-        if n == 0:
-            rf = {"eigenvalues": np.zeros(0), "vectors": None,
-                  "residuals": np.zeros(0), "rounds": 0, "matvecs": 0,
-                  "seconds": 0.0, "converged": False}
-        else:
-            rf = refine_eigenpairs(H, dv, r1["eigenvalues"], tol=tol,
-                                   hnorm=hnorm, depth=refine_depth,
-                                   max_rounds=refine_rounds)
+        # refined per side of the gap (valence levels, conduction levels):
+        # the two groups are far apart in energy, so one joint block helps
+        # neither and its Gram-Schmidt costs twice the two blocks'
+        ev32 = np.asarray(r1["eigenvalues"])
+        split = int(np.searchsorted(ev32, 0.5 * (erange_vb[1] + erange_cb[0])))
+        groups = [(a, b) for a, b in ((0, split), (split, n)) if b > a]
+        if not refine_split:
+            groups = [(0, n)] if n else []
+        rf = {"eigenvalues": np.zeros(n), "residuals": np.zeros(n), "rounds": 0,
+              "matvecs": 0, "seconds": 0.0, "converged": n > 0}
+        for a, b in groups:
+            g = refine_eigenpairs(H, dv, ev32[a:b], tol=tol, hnorm=hnorm,
+                                  depth=refine_depth, max_rounds=refine_rounds,
+                                  row0=a)
+            rf["eigenvalues"][a:b] = g["eigenvalues"]
+            rf["residuals"][a:b] = g["residuals"]
+            rf["rounds"] = max(rf["rounds"], g["rounds"])
+            rf["matvecs"] += g["matvecs"]
+            rf["seconds"] += g["seconds"]
+            rf["converged"] = rf["converged"] and g["converged"]
     finally:
         if prec0 is not None:
             H.set_precision(prec0)
