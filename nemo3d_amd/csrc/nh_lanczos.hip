@@ -116,12 +116,12 @@ struct LzBlockSum<true> {
 // separate update kernel and the H*v kernel of the eigenvector pass, which
 // fuses the step, round identically: the second pass must retrace the first
 // pass's trajectory bit for bit.
-__device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
-                                                 const double2 *rp, double al,
-                                                 double be, double invp) {
+__device__ __forceinline__ double2 lz_three_term_v(double2 y, double2 q,
+                                                   bool has_p, double2 r,
+                                                   double al, double be,
+                                                   double invp) {
   asm volatile("" : "+v"(y.x), "+v"(y.y));
-  if (rp) {
-    const double2 r = *rp;
+  if (has_p) {
     const double2 p = make_double2(r.x * invp, r.y * invp);
     y.x = y.x - __builtin_fma(al, q.x, be * p.x);
     y.y = y.y - __builtin_fma(al, q.y, be * p.y);
@@ -130,6 +130,13 @@ __device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
     y.y = __builtin_fma(-al, q.y, y.y);
   }
   return y;
+}
+// the same step with r_prev read from rp (null: first step)
+__device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
+                                                 const double2 *rp, double al,
+                                                 double be, double invp) {
+  return lz_three_term_v(y, q, rp != nullptr, rp ? *rp : make_double2(0.0, 0.0),
+                         al, be, invp);
 }
 
 template <int NSP, typename R, bool QUAD, bool FUSE>
@@ -183,6 +190,18 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
     auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
     const int la = threadIdx.x / 4;
     double2 own_x[NO];
+    // FUSE: r_prev of the lane's rows loaded before the epilogue's barriers,
+    // so the loads overlap the staging instead of trailing it
+    double2 rpv[NO];
+#pragma unroll
+    for (int i = 0; i < NO; ++i) rpv[i] = make_double2(0.0, 0.0);
+    if constexpr (FUSE) {
+      if (active && Rp) {
+#pragma unroll
+        for (int i = 0; i < NO; ++i)
+          if (4 * i + slot < NB) rpv[i] = Rp[(int64_t)atom * NB + 4 * i + slot];
+      }
+    }
     __syncthreads();  // every lane has read its staged neighbour entries
     if (active)
       sk_stage_own<NSP>(la, slot, Rv + (int64_t)atom * NB,
@@ -200,7 +219,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
         const int64_t e = (int64_t)atom * NB + r;
         double2 y = make_double2(re * inv, im * inv);
         if constexpr (FUSE) {
-          y = lz_three_term(y, q, Rp ? Rp + e : nullptr, al, be, invp);
+          y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
           if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
         }
         if (Q) Q[e] = q;
