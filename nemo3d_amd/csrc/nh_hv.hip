@@ -42,11 +42,14 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
   if constexpr (QUAD) {
     auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
     const int la = threadIdx.x / 4;
-    double2 own_x[NO];
-    __syncthreads();  // every lane has read its staged neighbour entries
+    // own entries and onsite record loaded before the barrier (the loads
+    // overlap it), staged in the freed LDS after it
+    SkOwnRegs<NSP> own;
     if (active)
-      sk_stage_own<NSP>(la, slot, xc + (int64_t)atom * NB,
-                        onsite + (size_t)atom * kOnsiteStride, S, own_x);
+      sk_load_own<NSP>(slot, xc + (int64_t)atom * NB,
+                       onsite + (size_t)atom * kOnsiteStride, own);
+    __syncthreads();  // every lane has read its staged neighbour entries
+    if (active) sk_store_own<NSP>(la, slot, own, S);
     __syncthreads();
     if (!active) return;
 #pragma unroll

@@ -277,25 +277,42 @@ struct SkOwnStage {
   double2 os[kSkAtomsPerBlock][kOnsiteStride / 2];
 };
 
+// Own entries and onsite record loaded into registers (before the barrier
+// that frees the staging LDS, so the loads overlap it) ...
 template <int NSP>
-__device__ __forceinline__ void sk_stage_own(int la, int slot,
-                                             const double2 *__restrict__ xa,
-                                             const double *__restrict__ os,
-                                             SkOwnStage<NSP> &S,
-                                             double2 (&own_x)[(NSP * 10 + 3) / 4]) {
+struct SkOwnRegs {
+  static constexpr int kOs = (kOnsiteStride / 2 + 3) / 4;  // onsite chunks per lane
+  double2 x[(NSP * 10 + 3) / 4];
+  double2 os[kOs];
+};
+template <int NSP>
+__device__ __forceinline__ void sk_load_own(int slot, const double2 *__restrict__ xa,
+                                            const double *__restrict__ os,
+                                            SkOwnRegs<NSP> &g) {
   constexpr int NB = NSP * 10;
 #pragma unroll
   for (int i = 0; i < (NB + 3) / 4; ++i) {
     const int r = 4 * i + slot;
-    if (r < NB) {
-      own_x[i] = xa[r];
-      S.x[la][r] = own_x[i];
-    }
+    g.x[i] = r < NB ? xa[r] : make_double2(0.0, 0.0);
   }
   const double2 *o2 = reinterpret_cast<const double2 *>(os);  // 144-byte records
 #pragma unroll
-  for (int c0 = 0; c0 < kOnsiteStride / 2; c0 += 4)
-    if (c0 + slot < kOnsiteStride / 2) S.os[la][c0 + slot] = o2[c0 + slot];
+  for (int i = 0; i < SkOwnRegs<NSP>::kOs; ++i) {
+    const int c = 4 * i + slot;
+    g.os[i] = c < kOnsiteStride / 2 ? o2[c] : make_double2(0.0, 0.0);
+  }
+}
+// ... and stored to the epilogue staging after it
+template <int NSP>
+__device__ __forceinline__ void sk_store_own(int la, int slot, const SkOwnRegs<NSP> &g,
+                                             SkOwnStage<NSP> &S) {
+  constexpr int NB = NSP * 10;
+#pragma unroll
+  for (int i = 0; i < (NB + 3) / 4; ++i)
+    if (4 * i + slot < NB) S.x[la][4 * i + slot] = g.x[i];
+#pragma unroll
+  for (int i = 0; i < SkOwnRegs<NSP>::kOs; ++i)
+    if (4 * i + slot < kOnsiteStride / 2) S.os[la][4 * i + slot] = g.os[i];
 }
 
 // Exchange of a double with lane ^ 1 / lane ^ 2 inside the quad through

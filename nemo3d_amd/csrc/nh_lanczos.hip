@@ -189,9 +189,13 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
                   "epilogue staging must fit the gather staging");
     auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
     const int la = threadIdx.x / 4;
-    double2 own_x[NO];
-    // FUSE: r_prev of the lane's rows loaded before the epilogue's barriers,
-    // so the loads overlap the staging instead of trailing it
+    // own entries, onsite record and (FUSE) r_prev of the lane's rows are
+    // loaded before the epilogue's barriers, so the loads overlap them; the
+    // first two are staged in the freed LDS after the first barrier
+    SkOwnRegs<NSP> own;
+    if (active)
+      sk_load_own<NSP>(slot, Rv + (int64_t)atom * NB,
+                       onsite + (size_t)atom * kOnsiteStride, own);
     double2 rpv[NO];
 #pragma unroll
     for (int i = 0; i < NO; ++i) rpv[i] = make_double2(0.0, 0.0);
@@ -203,9 +207,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
       }
     }
     __syncthreads();  // every lane has read its staged neighbour entries
-    if (active)
-      sk_stage_own<NSP>(la, slot, Rv + (int64_t)atom * NB,
-                        onsite + (size_t)atom * kOnsiteStride, S, own_x);
+    if (active) sk_store_own<NSP>(la, slot, own, S);
     __syncthreads();
     if (active) {
 #pragma unroll
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
         double re = own_re[i], im = own_im[i];
         onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la],
                         r / 10, r % 10, 0.0, re, im);
-        const double2 q = make_double2(own_x[i].x * inv, own_x[i].y * inv);
+        const double2 q = make_double2(own.x[i].x * inv, own.x[i].y * inv);
         const int64_t e = (int64_t)atom * NB + r;
         double2 y = make_double2(re * inv, im * inv);
         if constexpr (FUSE) {
