@@ -77,8 +77,9 @@ nh_status nh_context_create(const nh_model *model, int device,
 void nh_context_destroy(nh_context *ctx);
 int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
 
-/* Kernel variants (chosen by measured throughput; the same products, bit
- * for bit in FP64 and to rounding in FP32).  NH_SET_HV_STAGED: 1 stages the neighbours'
+/* Kernel variants (chosen by measured throughput; the same products to
+ * rounding: the compiler contracts the block's multiply-adds differently
+ * in the two).  NH_SET_HV_STAGED: 1 stages the neighbours'
  * vector entries in LDS (stored-block layout: per workgroup; SK layout: the
  * four lanes of an atom load each neighbour's entries together), 0 reads
  * them from the caches by every lane, -1 (default) picks by layout (SK:

@@ -193,10 +193,11 @@ def test_hv_staged_and_cached_variants_agree(case, precision):
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_hv_sk_quad_gather_matches_oracle(case, precision):
     """SK layout with the quad-cooperative LDS gather of the neighbours'
-    entries (and, in FP32, of the bond records): FP64 products equal to the
-    direct gather bit for bit, FP32 ones to FP32 rounding (the compiler
-    contracts multiply-adds differently in the two kernels), oracle to the
-    FP64 / FP32 bar, single and 3-column products."""
+    entries: products equal to the direct gather's to FP64 / FP32 rounding
+    (the compiler contracts the block's multiply-adds differently around
+    the staged loads), oracle to the FP64 / FP32 bar, single and 3-column
+    products (the single product equal to the block's column bit for
+    bit)."""
     M = make(case)
     print("case", case, precision, flush=True)
     ref = oracle.assemble(M.arrays())
@@ -208,11 +209,9 @@ def test_hv_sk_quad_gather_matches_oracle(case, precision):
     y1 = H.hv(X[1], 0.3)
     H.set(native.SET_HV_STAGED, 0)
     Yc = H.hV(X, 0.3)
-    if precision == "fp64":
-        assert np.array_equal(Ys, Yc) and np.array_equal(y1, Yc[1])
-    else:
-        assert np.linalg.norm(Ys - Yc) <= 1e-6 * np.linalg.norm(Yc)
-        assert np.array_equal(y1, Ys[1])
+    bar = 1e-14 if precision == "fp64" else 1e-6
+    assert np.linalg.norm(Ys - Yc) <= bar * np.linalg.norm(Yc)
+    assert np.array_equal(y1, Ys[1])
     tol = 1e-12 if precision == "fp64" else 2e-6
     for k in range(3):
         yr = ref.hv(X[k], 0.3)
@@ -222,8 +221,8 @@ def test_hv_sk_quad_gather_matches_oracle(case, precision):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_lanczos_sk_quad_gather_same_run(precision):
     """The fused Lanczos kernels with the quad LDS gather give the same
-    tridiagonal as the direct gather: bit for bit in FP64; in FP32 (products
-    equal to rounding) the same extreme Ritz values."""
+    tridiagonal as the direct gather up to the rounding of the products:
+    the same extreme Ritz values (to 1e-10 eV in FP64, 1e-5 eV in FP32)."""
     from nemo3d_amd.solvers import eigensolve_lanczos
     M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=23)
     H = Hamiltonian(M, layout=native.LAYOUT_SK,
@@ -235,11 +234,10 @@ def test_lanczos_sk_quad_gather_same_run(precision):
     H.set(native.SET_HV_STAGED, 1)
     r1 = eigensolve_lanczos(H, **opts)
     H.set(native.SET_HV_STAGED, -1)
-    if precision == "fp64":
-        assert np.array_equal(r0["alpha"], r1["alpha"])
-        assert np.array_equal(r0["beta"], r1["beta"])
-    else:
-        from scipy.linalg import eigh_tridiagonal
-        w0 = eigh_tridiagonal(r0["alpha"], r0["beta"][:-1], eigvals_only=True)
-        w1 = eigh_tridiagonal(r1["alpha"], r1["beta"][:-1], eigvals_only=True)
-        assert np.abs(w0[[0, 1, -2, -1]] - w1[[0, 1, -2, -1]]).max() < 1e-5
+    from scipy.linalg import eigh_tridiagonal
+    w0 = eigh_tridiagonal(r0["alpha"], r0["beta"][:-1], eigvals_only=True)
+    w1 = eigh_tridiagonal(r1["alpha"], r1["beta"][:-1], eigvals_only=True)
+    bar = 1e-10 if precision == "fp64" else 1e-5
+    assert np.abs(w0[[0, 1, -2, -1]] - w1[[0, 1, -2, -1]]).max() < bar
+    if precision == "fp64":  # the first steps agree to rounding
+        assert np.abs(r0["alpha"][:20] - r1["alpha"][:20]).max() < 1e-12
