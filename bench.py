@@ -218,8 +218,8 @@ def main():
         start = None
         if net is not None:  # network inference + start vector, timed
             blk = warmstart.start_block(net, H, inputs=inputs, device=True)
-            start = warmstart.start_vector_device(H, blk)
-            del blk
+            # the start vector stays in device memory (row 0 of the block)
+            start = warmstart.start_vector_device(H, blk, download=False)
         t_cnn = time.time() - t
         r = mixed_solve(H, start, args.max_iter, windows, hnorm)
         r["cnn_s"] = t_cnn

@@ -154,10 +154,13 @@ typedef struct nh_lanczos_opts {
   double erange_cb[2];          /* Erange_cb */
   int32_t conv_check_start;     /* ConvCheckStartIter */
   int32_t conv_check_skip;      /* ConvCheckSkipRate */
-  int32_t start;                /* 0 CyclicRandom, 1 Const1, 2 user vector */
+  int32_t start;                /* 0 CyclicRandom, 1 Const1, 2 user vector
+                                   (host), 3 user vector (device); built and
+                                   normalised in device memory */
   int32_t cyclic_len;           /* CyclicString length */
   const double *cyclic_string;  /* CyclicString values */
-  const nh_cplx *start_vector;  /* start == 2: host vector of length N */
+  const nh_cplx *start_vector;  /* start == 2 / 3: host / device vector of
+                                   length N */
   int32_t stop_rule;            /* 0: stop once num_eig converged values lie
                                    in each window (lanczos.c:1231-1232);
                                    1: stop once the num_eig Ritz values

@@ -555,7 +555,9 @@ class LanczosRun {
   // r_{k+1} to slot k + 1, and the accumulation reads them as its ring
   // (1 / beta_{k-1} folded into the coefficients)
   float2 *ring32_ = nullptr;
-  nh_status upload_start(const std::vector<nh_cplx> &r0, float2 *f32);
+  nh_status build_start(const nh_lanczos_opts *o);
+  nh_status upload_start(float2 *f32);
+  double2 *R0_ = nullptr;  // the normalised start vector (device)
   double *alpha_ = nullptr, *beta_ = nullptr, *part_ = nullptr;
   int nparts_ = 0;
   std::vector<double> ha_, hb_;
@@ -605,16 +607,67 @@ __global__ __launch_bounds__(256) void k_to_f32(int64_t n,
     y[k] = make_float2((float)x[k].x, (float)x[k].y);
 }
 
+// start vector (lanczos_initialize, eigsys3d_par.c:113-215) built and
+// normalised in device memory: CyclicRandom r0[j] = (s[2j mod L],
+// s[2j+1 mod L]), Const1 (1, 1), or a caller's host or device vector
+__global__ __launch_bounds__(256) void k_start_fill(int64_t n, int kind, int L,
+                                                   const double *__restrict__ cyc,
+                                                   double2 *__restrict__ r) {
+  for (int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * 256)
+    r[j] = kind == 0 ? make_double2(cyc[(2 * j) % L], cyc[(2 * j + 1) % L])
+                     : make_double2(1.0, 1.0);
+}
+__global__ __launch_bounds__(256) void k_div(int64_t n, double2 *__restrict__ r,
+                                             double s) {
+  for (int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * 256)
+    r[j] = make_double2(r[j].x / s, r[j].y / s);
+}
+
+nh_status LanczosRun::build_start(const nh_lanczos_opts *o) {
+  NH_TRY(alloc((void **)&R0_, n_ * sizeof(double2)));
+  const int g = reduce_blocks(n_);
+  if (o->start == 0 || o->start == 1) {
+    double *cyc = nullptr;
+    int L = 1;
+    if (o->start == 0) {
+      NH_CHECK(o->cyclic_len > 0 && o->cyclic_string, NH_ERR_ARG,
+               "CyclicRandom start needs CyclicString");
+      L = o->cyclic_len;
+      NH_TRY(alloc((void **)&cyc, (size_t)L * sizeof(double)));
+      NH_HIP(hipMemcpyAsync(cyc, o->cyclic_string, (size_t)L * sizeof(double),
+                            hipMemcpyHostToDevice, c_->stream));
+    }
+    hipLaunchKernelGGL(k_start_fill, dim3(g), dim3(256), 0, c_->stream, n_,
+                       o->start, L, cyc, R0_);
+    NH_HIP(hipGetLastError());
+    // the host CyclicString may be released on return
+    NH_HIP(hipStreamSynchronize(c_->stream));
+  } else {
+    NH_CHECK(o->start == 2 || o->start == 3, NH_ERR_ARG, "bad start %d", o->start);
+    NH_CHECK(o->start_vector, NH_ERR_ARG, "start vector missing");
+    NH_HIP(hipMemcpyAsync(R0_, o->start_vector, n_ * sizeof(double2),
+                          o->start == 2 ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                          c_->stream));
+  }
+  double s = 0.0;
+  NH_TRY(reduce_nrm2sq(c_, n_, reinterpret_cast<const nh_cplx *>(R0_), &s));
+  NH_CHECK(s > 0, NH_ERR_ARG, "zero start vector");
+  hipLaunchKernelGGL(k_div, dim3(g), dim3(256), 0, c_->stream, n_, R0_, std::sqrt(s));
+  NH_HIP(hipGetLastError());
+  return NH_OK;
+}
+
 // r <- r0 (and its FP32 copy f32 when the products are FP32)
-nh_status LanczosRun::upload_start(const std::vector<nh_cplx> &r0, float2 *f32) {
-  NH_HIP(hipMemcpyAsync(R_, r0.data(), n_ * sizeof(double2),
-                        hipMemcpyHostToDevice, c_->stream));
+nh_status LanczosRun::upload_start(float2 *f32) {
+  NH_HIP(hipMemcpyAsync(R_, R0_, n_ * sizeof(double2), hipMemcpyDeviceToDevice,
+                        c_->stream));
   if (f32) {
     hipLaunchKernelGGL(k_to_f32, dim3(reduce_blocks(n_)), dim3(256), 0,
                        c_->stream, n_, R_, f32);
     NH_HIP(hipGetLastError());
   }
-  // the host vector r0 lives on: the copy may still be in flight
   return NH_OK;
 }
 
@@ -844,35 +897,12 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
                      2 * kRedMaxBlocks) + 64;
   NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * sizeof(double)));
 
-  // start vector (lanczos_initialize, eigsys3d_par.c:113-215)
-  std::vector<nh_cplx> r0(n_);
-  if (o->start == 0) {
-    NH_CHECK(o->cyclic_len > 0 && o->cyclic_string, NH_ERR_ARG,
-             "CyclicRandom start needs CyclicString");
-    const int L = o->cyclic_len;
-    for (int64_t j = 0; j < n_; ++j) {
-      r0[j].re = o->cyclic_string[(2 * j) % L];
-      r0[j].im = o->cyclic_string[(2 * j + 1) % L];
-    }
-  } else if (o->start == 1) {
-    for (int64_t j = 0; j < n_; ++j) r0[j].re = r0[j].im = 1.0;
-  } else {
-    NH_CHECK(o->start_vector, NH_ERR_ARG, "start vector missing");
-    for (int64_t j = 0; j < n_; ++j) r0[j] = o->start_vector[j];
-  }
-  double s = 0.0;
-  for (int64_t j = 0; j < n_; ++j) s += r0[j].re * r0[j].re + r0[j].im * r0[j].im;
-  NH_CHECK(s > 0, NH_ERR_ARG, "zero start vector");
-  s = std::sqrt(s);
-  for (int64_t j = 0; j < n_; ++j) {
-    r0[j].re /= s;
-    r0[j].im /= s;
-  }
+  NH_TRY(build_start(o));
   if (c->layout == NH_LAYOUT_SK && c->precision == NH_PREC_FP32) {
     NH_TRY(alloc((void **)&R32_, n_ * sizeof(float2)));
     NH_TRY(alloc((void **)&Y32_, n_ * sizeof(float2)));
   }
-  NH_TRY(upload_start(r0, R32_));
+  NH_TRY(upload_start(R32_));
 
   // pass 1: eigenvalues.  The host analyses the tridiagonal of one check
   // point while the device runs the steps up to the next one: at a check
@@ -988,7 +1018,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     }
     else
       NH_TRY(alloc((void **)&ring_, (size_t)ring_k_ * n_ * sizeof(double2)));
-    NH_TRY(upload_start(r0, ring32_));
+    NH_TRY(upload_start(ring32_));
     std::vector<double> hz((size_t)kMaxVec * kRingMax);
     const int rk = ring_k_;
     for (int k = 0; k < mmax; ++k) {

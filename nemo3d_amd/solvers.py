@@ -14,7 +14,7 @@ import numpy as np
 from . import native
 
 DEFAULT_CYCLIC = (1.2, 0.8, -1.4, 0.1, -0.9, -0.2, 0.6)
-START = {"CyclicRandom": 0, "Const1": 1, "Vector": 2}
+START = {"CyclicRandom": 0, "Const1": 1, "Vector": 2, "DeviceVector": 3}
 
 
 def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
@@ -38,11 +38,17 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     o.erange_cb[0], o.erange_cb[1] = map(float, erange_cb)
     o.conv_check_start = int(start_iter)
     o.conv_check_skip = int(skip)
-    o.start = START[start if start_vector is None else "Vector"]
     cyc = np.ascontiguousarray(cyclic, dtype=np.float64)
     o.cyclic_len = cyc.size
     o.cyclic_string = native.ptr(cyc)
-    if start_vector is not None:
+    if start_vector is None:
+        o.start = START[start]
+    elif isinstance(start_vector, native.DeviceArray):  # stays on the device
+        assert start_vector.shape[-1] == H.n and start_vector.dtype == np.complex128
+        o.start = START["DeviceVector"]
+        o.start_vector = start_vector.ptr
+    else:
+        o.start = START["Vector"]
         sv = np.ascontiguousarray(start_vector, dtype=np.complex128)
         o.start_vector = native.ptr(sv)
     o.stop_rule = {"window": 0, "nearest": 1}[stop_rule]

@@ -151,10 +151,12 @@ def start_block(net, H, n_states=learned.N_STATES, inputs=None, device=False):
     return net.start_block(f, g, n_states, device=device)
 
 
-def start_vector_device(H, block):
+def start_vector_device(H, block, download=True):
     """start_vector of a device block (native.DeviceArray, n_states x N): the
-    sum of the normalised columns formed on the device, returned on the host
-    (N values).  The block's first column is overwritten."""
+    sum of the normalised columns formed on the device in the block's first
+    column (overwritten).  Returned on the host (N values), or with
+    download=False the block itself, whose first row the Lanczos solver
+    takes as a device start vector."""
     lib = native.load()
     k, n = (int(v) for v in block.shape)
     base = int(block.ptr.value)
@@ -168,6 +170,8 @@ def start_vector_device(H, block):
             native.check(lib.nh_axpby(H.ctx, n, zero, col[0], s, col[0]))
         else:
             native.check(lib.nh_axpby(H.ctx, n, s, col[c], one, col[0]))
+    if not download:
+        return block
     out = np.empty(n, dtype=np.complex128)
     native.check(lib.nh_memcpy_d2h(H.ctx, native.ptr(out), col[0], n * 16))
     return out

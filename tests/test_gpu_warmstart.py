@@ -74,6 +74,16 @@ def test_learned_start_gains_over_random():
     sv_host = warmstart.start_vector(block)
     sv_dev = warmstart.start_vector_device(H, warmstart.start_block(net, H, device=True))
     assert np.linalg.norm(sv_dev - sv_host) <= 1e-12 * np.linalg.norm(sv_host)
+    # a device start vector (start = 3) gives the run of the same host vector
+    blk = warmstart.start_block(net, H, device=True)
+    dev = warmstart.start_vector_device(H, blk, download=False)
+    host = dev.download()[0]
+    opts = dict(num_vb=0, num_cb=1, erange_cb=(1e3, 1e3 + 1), max_iter=60,
+                start_iter=10 ** 6, want_vectors=False, return_tridiag=True)
+    r_d = eigensolve_lanczos(H, start_vector=dev, **opts)
+    r_h = eigensolve_lanczos(H, start_vector=host, **opts)
+    assert np.array_equal(r_d["alpha"], r_h["alpha"])
+    assert np.array_equal(r_d["beta"], r_h["beta"])
 
     m_rand = iterations_to_converge(None)
     m_learn = iterations_to_converge(warmstart.start_vector(block))
