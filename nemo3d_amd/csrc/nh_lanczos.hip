@@ -555,6 +555,11 @@ class LanczosRun {
   // r_{k+1} to slot k + 1, and the accumulation reads them as its ring
   // (1 / beta_{k-1} folded into the coefficients)
   float2 *ring32_ = nullptr;
+  // eigenvector pass with FP64 SK products: the r vectors themselves live in
+  // a ring of 2 K FP64 slots (slot k % (2 K) holds r_k: read as own entries
+  // and gathered at step k, r_(k-1) from slot k - 1, r_(k+1) written to slot
+  // k + 1), so no q is stored; the accumulation reads them like the FP32 ring
+  double2 *ringr_ = nullptr;
   nh_status build_start(const nh_lanczos_opts *o);
   nh_status upload_start(float2 *f32);
   double2 *R0_ = nullptr;  // the normalised start vector (device)
@@ -679,7 +684,14 @@ nh_status LanczosRun::iterate(int k, bool red) {
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   // q_k is stored (ring slot) only in the eigenvector pass, which
   // accumulates from it; the update forms q_k, q_{k-1} from r_k, r_{k-1}
-  double2 *Q = red || ring32_ ? nullptr : ring_ + (int64_t)(k % ring_k_) * n_;
+  double2 *Q = red || ring32_ || ringr_ ? nullptr : ring_ + (int64_t)(k % ring_k_) * n_;
+  // FP64 vectors of the step: r_k, r_(k-1), r_(k+1)
+  double2 *Rk = R_, *Rkm1 = Rm1_, *Yk = Y_;
+  if (!red && ringr_) {
+    Rk = ringr_ + (int64_t)(k % (2 * ring_k_)) * n_;
+    Rkm1 = ringr_ + (int64_t)((k + 2 * ring_k_ - 1) % (2 * ring_k_)) * n_;
+    Yk = ringr_ + (int64_t)((k + 1) % (2 * ring_k_)) * n_;
+  }
   // FP32 copies: gathered (r_k) and written (r_{k+1})
   const float2 *g32 = R32_;
   float2 *y32 = Y32_;
@@ -714,8 +726,8 @@ nh_status LanczosRun::iterate(int k, bool red) {
   hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE>), dim3(gridA),             \
                      dim3(kSkThreads), 0, c->stream, c->n_cations, c->cations,  \
                      c->n_anions, c->anions, ppx, c->onsite, SKP, c->nbr,       \
-                     c->bond_of, R_, g32, Q, Y_, bprev, part, alpha_ + k,       \
-                     k > 0 ? Rm1_ : nullptr, bprev2, y32)
+                     c->bond_of, Rk, g32, Q, Yk, bprev, part, alpha_ + k,       \
+                     k > 0 ? Rkm1 : nullptr, bprev2, y32)
 #define LZ_SK1(NSP, R, SKP, QUAD)                                               \
   do {                                                                          \
     if (fuse) LZ_SK2(NSP, R, SKP, QUAD, true);                                  \
@@ -1004,21 +1016,30 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
     NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRingMax * sizeof(double)));
     NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
-    // ring: 64 slots when they take at most a quarter of the free memory
-    // (each flush then reads and writes the eigenvectors once per 64 steps)
+    // ring: K = 64 steps per flush when the ring takes at most a quarter of
+    // the free memory, else 32, 16 or 8 (each flush reads and writes the
+    // eigenvectors once)
+    const bool rring = !R32_ && c->layout == NH_LAYOUT_SK;
     {
       size_t fr = 0, tot = 0;
       NH_HIP(hipMemGetInfo(&fr, &tot));
-      const size_t slot = R32_ ? 2 * n_ * sizeof(float2) : n_ * sizeof(double2);
-      ring_k_ = (size_t)kRingMax * slot <= fr / 4 ? kRingMax : kRingMax / 2;
+      const size_t slot = R32_ ? 2 * n_ * sizeof(float2)
+                               : (rring ? 2 : 1) * n_ * sizeof(double2);
+      ring_k_ = kRingMax;
+      while (ring_k_ > 8 && (size_t)ring_k_ * slot > fr / 4) ring_k_ /= 2;
     }
     if (R32_) {
       NH_CHECK(n_ % 2 == 0, NH_ERR_STATE, "Lanczos: FP32 ring needs an even N");
       NH_TRY(alloc((void **)&ring32_, (size_t)2 * ring_k_ * n_ * sizeof(float2)));
-    }
-    else
+      NH_TRY(upload_start(ring32_));
+    } else if (rring) {
+      NH_TRY(alloc((void **)&ringr_, (size_t)2 * ring_k_ * n_ * sizeof(double2)));
+      NH_HIP(hipMemcpyAsync(ringr_, R0_, n_ * sizeof(double2),
+                            hipMemcpyDeviceToDevice, c->stream));
+    } else {
       NH_TRY(alloc((void **)&ring_, (size_t)ring_k_ * n_ * sizeof(double2)));
-    NH_TRY(upload_start(ring32_));
+      NH_TRY(upload_start(nullptr));
+    }
     std::vector<double> hz((size_t)kMaxVec * kRingMax);
     const int rk = ring_k_;
     for (int k = 0; k < mmax; ++k) {
@@ -1031,8 +1052,8 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
         for (int e = 0; e < ne; ++e)
           for (int t = 0; t < K; ++t) {
             const int j = k0 + t;
-            // q_j = r_j / beta_{j-1} for the FP32 ring of r copies
-            const double sc = ring32_ && j > 0 ? 1.0 / hb_[j - 1] : 1.0;
+            // q_j = r_j / beta_{j-1} for the rings of r (FP32 copies or FP64)
+            const double sc = (ring32_ || ringr_) && j > 0 ? 1.0 / hb_[j - 1] : 1.0;
             hz[e * K + t] = j < pick[e].iter ? z[e][j] * sc : 0.0;
           }
         NH_HIP(hipMemcpyAsync(dZ, hz.data(), (size_t)ne * K * sizeof(double),
@@ -1045,6 +1066,10 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
           hipLaunchKernelGGL(k_lz_accum32, dim3((int)g), dim3(256), 0,
                              c->stream, n_, ne, K, dZ,
                              ring32_ + (int64_t)(k0 % (2 * rk)) * n_, n_, V, n_);
+        else if (ringr_)
+          hipLaunchKernelGGL((k_lz_accum<double2>), dim3((int)g), dim3(256), 0,
+                             c->stream, n_, ne, K, dZ,
+                             ringr_ + (int64_t)(k0 % (2 * rk)) * n_, n_, V, n_);
         else
           hipLaunchKernelGGL((k_lz_accum<double2>), dim3((int)g), dim3(256), 0,
                              c->stream, n_, ne, K, dZ, ring_, n_, V, n_);
