@@ -37,6 +37,54 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
+nh_status pool_alloc(nh_context *c, void **p, size_t bytes) {
+  bytes = bytes > 0 ? (bytes + 255) / 256 * 256 : 256;
+  int best = -1;
+  for (int i = 0; i < (int)c->pool.size(); ++i) {
+    const auto &b = c->pool[i];
+    if (!b.used && b.bytes >= bytes && b.bytes <= 2 * bytes &&
+        (best < 0 || b.bytes < c->pool[best].bytes))
+      best = i;
+  }
+  if (best >= 0) {
+    c->pool[best].used = true;
+    *p = c->pool[best].p;
+    return NH_OK;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    // return the free blocks to the device and try once more
+    (void)hipGetLastError();
+    NH_HIP(hipStreamSynchronize(c->stream));
+    std::vector<nh_context::PoolBlock> keep;
+    for (auto &b : c->pool) {
+      if (b.used) keep.push_back(b);
+      else NH_HIP(hipFree(b.p));
+    }
+    c->pool.swap(keep);
+    e = hipMalloc(p, bytes);
+  }
+  NH_CHECK(e == hipSuccess, NH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes,
+           hipGetErrorString(e));
+  c->pool.push_back({*p, bytes, true});
+  return NH_OK;
+}
+
+void pool_release(nh_context *c, void *p) {
+  for (auto &b : c->pool)
+    if (b.p == p) {
+      b.used = false;
+      return;
+    }
+}
+
+size_t pool_free_bytes(const nh_context *c) {
+  size_t s = 0;
+  for (const auto &b : c->pool)
+    if (!b.used) s += b.bytes;
+  return s;
+}
+
 nh_status ensure_buffers(nh_context *c, int64_t elems) {
   if (elems <= c->buf_cap) return NH_OK;
   hipFree(c->xbuf);
@@ -66,6 +114,8 @@ static void destroy(nh_context *c) {
                   c->ybuf};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  hipStreamSynchronize(c->stream);
+  for (auto &b : c->pool) hipFree(b.p);
   if (c->hred) hipHostFree(c->hred);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

@@ -26,7 +26,7 @@ class BlockOps {
   BlockOps(nh_context *c, int max_cols, int max_x = 0)
       : c_(c), n_(c->n), cap_(max_cols), maxx_(std::max(max_x, max_cols)) {}
   ~BlockOps() {
-    for (void *q : mem_) hipFree(q);
+    for (void *q : mem_) pool_release(c_, q);
   }
   nh_status setup() {
     NH_CHECK(cap_ >= 1 && cap_ <= 16, NH_ERR_ARG, "block width %d", cap_);
@@ -41,9 +41,10 @@ class BlockOps {
     NH_TRY(alloc((void **)&S_, (size_t)capp * n_ * sizeof(double2)));
     return NH_OK;
   }
-  // device memory owned by this object, zero-filled
+  // device memory owned by this object (from the context's workspace
+  // pool), zero-filled
   nh_status alloc(void **q, size_t bytes) {
-    NH_HIP(hipMalloc(q, bytes));
+    NH_TRY(pool_alloc(c_, q, bytes));
     mem_.push_back(*q);
     NH_HIP(hipMemsetAsync(*q, 0, bytes, c_->stream));
     return NH_OK;

@@ -100,6 +100,15 @@ struct nh_context {
   nh_cplx *xbuf = nullptr;      // host-pointer staging
   nh_cplx *ybuf = nullptr;
   int64_t buf_cap = 0;          // complex elements in xbuf/ybuf
+  // solver workspace kept across solves: blocks a solver releases stay
+  // allocated for the next request (a hipFree of tens of GB can stall for
+  // a second or more); freed with the context
+  struct PoolBlock {
+    void *p;
+    size_t bytes;
+    bool used;
+  };
+  std::vector<PoolBlock> pool;
 };
 
 namespace nh {
@@ -183,5 +192,13 @@ nh_status reduce_nrm2sq(nh_context *c, int64_t n, const nh_cplx *x,
 nh_status launch_axpby(nh_context *c, int64_t n, nh_cplx a, const nh_cplx *x,
                        nh_cplx b, nh_cplx *y);
 nh_status ensure_reduce(nh_context *c, int blocks);
+// device memory from the context's workspace pool: the smallest free block
+// of at least `bytes` and at most twice that, else a new allocation (after
+// returning the free blocks to the device if it is full); pool_release
+// hands a block back (work queued on the context's stream that uses it may
+// still be running: the next user is ordered after it on that stream)
+nh_status pool_alloc(nh_context *c, void **p, size_t bytes);
+void pool_release(nh_context *c, void *p);
+size_t pool_free_bytes(const nh_context *c);  // bytes in free blocks
 nh_status ensure_buffers(nh_context *c, int64_t elems);
 }  // namespace nh

@@ -524,15 +524,16 @@ class LanczosRun {
   LanczosRun(nh_context *c) : c_(c) {}
   ~LanczosRun() {
     release_events();
-    for (void *p : allocs_) hipFree(p);
+    for (void *p : allocs_) pool_release(c_, p);
     if (pin_) hipHostFree(pin_);
     if (fetch_ev_) hipEventDestroy(fetch_ev_);
   }
   nh_status run(const nh_lanczos_opts *o, nh_eig_result *res);
 
  private:
+  // from the context's workspace pool, handed back when the run ends
   nh_status alloc(void **p, size_t bytes) {
-    NH_HIP(hipMalloc(p, bytes));
+    NH_TRY(pool_alloc(c_, p, bytes));
     allocs_.push_back(*p);
     return NH_OK;
   }
@@ -1016,17 +1017,19 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
     NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRingMax * sizeof(double)));
     NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
-    // ring: K = 64 steps per flush when the ring takes at most a quarter of
-    // the free memory, else 32, 16 or 8 (each flush reads and writes the
+    // ring: K = 64 steps per flush when the ring takes at most half the
+    // free memory (the run's buffers are released before any later stage
+    // allocates), else 32, 16 or 8 (each flush reads and writes the
     // eigenvectors once)
     const bool rring = !R32_ && c->layout == NH_LAYOUT_SK;
     {
       size_t fr = 0, tot = 0;
       NH_HIP(hipMemGetInfo(&fr, &tot));
+      fr += pool_free_bytes(c);
       const size_t slot = R32_ ? 2 * n_ * sizeof(float2)
                                : (rring ? 2 : 1) * n_ * sizeof(double2);
       ring_k_ = kRingMax;
-      while (ring_k_ > 8 && (size_t)ring_k_ * slot > fr / 4) ring_k_ /= 2;
+      while (ring_k_ > 8 && (size_t)ring_k_ * slot > fr / 2) ring_k_ /= 2;
     }
     if (R32_) {
       NH_CHECK(n_ % 2 == 0, NH_ERR_STATE, "Lanczos: FP32 ring needs an even N");

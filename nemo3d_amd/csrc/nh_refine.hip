@@ -33,7 +33,7 @@ class Refine {
     n_ = c->n;
   }
   ~Refine() {
-    for (void *q : mem_) hipFree(q);
+    for (void *q : mem_) pool_release(c_, q);
   }
   nh_status setup();
   nh_status round(std::vector<double> &target, std::vector<double> &resid,
@@ -41,8 +41,9 @@ class Refine {
   double2 *X() { return X_; }
 
  private:
+  // from the context's workspace pool, zero-filled
   nh_status alloc(void **q, size_t bytes) {
-    NH_HIP(hipMalloc(q, bytes));
+    NH_TRY(pool_alloc(c_, q, bytes));
     mem_.push_back(*q);
     NH_HIP(hipMemsetAsync(*q, 0, bytes, c_->stream));
     return NH_OK;
