@@ -74,6 +74,9 @@ typedef struct nh_context nh_context;
 /* Context: copies the model to the device (setup_hamiltonian's outputs). */
 nh_status nh_context_create(const nh_model *model, int device,
                             nh_context **out);
+/* Releases the context, including the solvers' device workspace: buffers a
+ * solve releases stay allocated in the context for the next solve (a free
+ * of tens of GB can stall for a second or more). */
 void nh_context_destroy(nh_context *ctx);
 int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
 
