@@ -88,17 +88,29 @@ def build_dot(workload="dot786k"):
     return M, info
 
 
-def lanczos_memory(n, k=8):
-    """Device bytes of the Lanczos vectors at N = n: the eigenvalue pass
-    keeps r_k, r_(k-1) and y in FP64 (+ two FP32 copies of r with FP32
-    products); the eigenvector pass adds the ring (64 FP64 slots, or 2 x 64
-    FP32 copies; 32 when memory is short) and the k FP64 eigenvectors."""
+def lanczos_memory(n, free_bytes=288e9, k=8):
+    """Device bytes of the Lanczos vectors at N = n (SK layout): the
+    eigenvalue pass keeps r_k, r_(k-1) and y in FP64 (+ two FP32 copies of r
+    with FP32 products); the eigenvector pass adds the ring of r vectors
+    (2 K FP64 slots, or 2 K FP32 copies with FP32 products; K = 64 steps
+    per flush, halved down to 8 until the ring takes at most half the free
+    memory) and the k FP64 eigenvectors."""
     v64, v32 = n * 16, n * 8
+
+    def ring(per_step):  # bytes per step of K: two slots
+        K = 64
+        while K > 8 and K * per_step > free_bytes / 2:
+            K //= 2
+        return K, K * per_step
+
+    k64, r64 = ring(2 * v64)
+    k32, r32 = ring(2 * v32)
     return {"eigenvalue_pass_fp64_GB": 3 * v64 / 1e9,
             "eigenvalue_pass_fp32_products_GB": (3 * v64 + 2 * v32) / 1e9,
-            "eigenvector_pass_fp64_GB": (3 * v64 + 64 * v64 + k * v64) / 1e9,
-            "eigenvector_pass_fp32_products_GB":
-                (3 * v64 + 2 * v32 + 128 * v32 + k * v64) / 1e9}
+            "eigenvector_pass_fp64_GB": (3 * v64 + r64 + k * v64) / 1e9,
+            "eigenvector_pass_fp64_ring_steps": k64,
+            "eigenvector_pass_fp32_products_GB": (3 * v64 + 2 * v32 + r32 + k * v64) / 1e9,
+            "eigenvector_pass_fp32_ring_steps": k32}
 
 
 def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False):
