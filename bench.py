@@ -174,7 +174,9 @@ def plain_solve(H, max_iter, windows=WINDOWS):
 # rounding of H·v, ~1e-7 ||H||); one FP64 refinement round takes them to
 # 1e-8.  Measured on the 786k dot: 1e-8 6.82 s, 3e-8 6.65 s, 1e-7 6.82 s
 # (two rounds), 3e-7 not converged after four rounds (neighbouring levels
-# outside the refinement's block).
+# outside the refinement's block); with the refinement split by side of the
+# gap: 3e-8 5.94 s, 5e-8 5.90 s (residuals up to 9.3e-9, little margin),
+# 7e-8 5.98 s (two rounds).
 TOL32 = 3e-8
 
 
