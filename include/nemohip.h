@@ -88,6 +88,11 @@ int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
  * them from the caches by every lane, -1 (default) picks by layout (SK:
  * staged, stored blocks: direct). */
 enum { NH_SET_HV_STAGED = 1 };
+/* NH_SET_RING_STEPS: Lanczos steps between eigenvector accumulations in the
+ * eigenvector pass (ring length; 8, 16, 32 or 64), or 0 (default): 64,
+ * halved down to 8 until the ring takes at most half the free memory.
+ * The eigenvectors agree to rounding whatever the length. */
+enum { NH_SET_RING_STEPS = 2 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 
 /* Assemble H on the device in the given layout and precision.  Replaces

@@ -81,6 +81,7 @@ struct nh_context {
   // atom dot: faster for the plain product, slower inside the fused Lanczos
   // kernel, so off by default)
   int hv_staged = -1;  // NH_SET_HV_STAGED (-1: by layout, see use_staged)
+  int ring_steps = 0;  // NH_SET_RING_STEPS (0: by free memory)
 
   // assembled Hamiltonian
   int layout = -1;

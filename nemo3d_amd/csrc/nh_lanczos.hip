@@ -1030,6 +1030,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
                                : (rring ? 2 : 1) * n_ * sizeof(double2);
       ring_k_ = kRingMax;
       while (ring_k_ > 8 && (size_t)ring_k_ * slot > fr / 2) ring_k_ /= 2;
+      if (c->ring_steps > 0) ring_k_ = std::min(ring_k_, c->ring_steps);
     }
     if (R32_) {
       NH_CHECK(n_ % 2 == 0, NH_ERR_STATE, "Lanczos: FP32 ring needs an even N");

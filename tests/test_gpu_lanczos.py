@@ -236,6 +236,37 @@ def test_fp32_lanczos_vectors_at_fp32_level():
           flush=True)
 
 
+@pytest.mark.parametrize("layout,precision", [("sk", "fp64"), ("sk", "fp32"),
+                                               ("blocks", "fp64")])
+def test_eigenvector_ring_length(layout, precision):
+    """The eigenvector pass adds its ring into the eigenvectors every K
+    steps (K = 64, fewer when HBM is short): K = 8 gives the same
+    eigenpairs to rounding (SK: rings of r, FP64 or FP32 copies; stored
+    blocks: ring of q)."""
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=24)
+    ref = oracle.assemble(M.arrays())
+    w = distinct(np.linalg.eigvalsh(ref.dense()))
+    wv, wc = gap_windows(w)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK if layout == "sk" else native.LAYOUT_BLOCKS,
+                    precision=native.PREC_FP64 if precision == "fp64" else native.PREC_FP32)
+    out = {}
+    for K in (0, 8):
+        H.set(native.SET_RING_STEPS, K)
+        out[K] = eigensolve_lanczos(H, num_vb=2, num_cb=2, erange_vb=wv,
+                                    erange_cb=wc, max_iter=6000, tol=1e-7,
+                                    start_iter=100, skip=25,
+                                    return_vectors=True, stop_rule="nearest")
+    H.set(native.SET_RING_STEPS, 0)
+    a, b = out[0], out[8]
+    print(layout, precision, "iterations", a["iterations"], "eigenvalues",
+          a["eigenvalues"], flush=True)
+    assert a["converged"] and b["converged"]
+    assert a["iterations"] == b["iterations"]
+    assert np.abs(a["eigenvalues"] - b["eigenvalues"]).max() < 1e-12
+    for va, vb in zip(a["vectors"], b["vectors"]):
+        assert np.linalg.norm(va - vb) < 1e-10 * np.linalg.norm(va)
+
+
 def test_refine_eigenpairs_from_perturbed_vectors():
     """FP64 refinement of dense eigenvectors perturbed by noise: residuals
     fall below the tolerance and the values reach the dense ones."""
