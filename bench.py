@@ -92,13 +92,13 @@ def lanczos_memory(n, free_bytes=288e9, k=8):
     """Device bytes of the Lanczos vectors at N = n (SK layout): the
     eigenvalue pass keeps r_k, r_(k-1) and y in FP64 (+ two FP32 copies of r
     with FP32 products); the eigenvector pass adds the ring of r vectors
-    (2 K FP64 slots, or 2 K FP32 copies with FP32 products; K = 64 steps
+    (2 K FP64 slots, or 2 K FP32 copies with FP32 products; K = 128 steps
     per flush, halved down to 8 until the ring takes at most half the free
     memory) and the k FP64 eigenvectors."""
     v64, v32 = n * 16, n * 8
 
     def ring(per_step):  # bytes per step of K: two slots
-        K = 64
+        K = 128
         while K > 8 and K * per_step > free_bytes / 2:
             K //= 2
         return K, K * per_step

@@ -89,7 +89,7 @@ int64_t nh_context_dim(const nh_context *ctx); /* N = n_atoms * nb */
  * staged, stored blocks: direct). */
 enum { NH_SET_HV_STAGED = 1 };
 /* NH_SET_RING_STEPS: Lanczos steps between eigenvector accumulations in the
- * eigenvector pass (ring length; 8, 16, 32 or 64), or 0 (default): 64,
+ * eigenvector pass (ring length; 8, 16, 32, 64 or 128), or 0 (default): 128,
  * halved down to 8 until the ring takes at most half the free memory.
  * The eigenvectors agree to rounding whatever the length. */
 enum { NH_SET_RING_STEPS = 2 };

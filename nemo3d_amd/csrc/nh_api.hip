@@ -276,8 +276,9 @@ nh_status nh_context_set(nh_context *ctx, int what, int value) {
       ctx->hv_staged = value;
       return NH_OK;
     case NH_SET_RING_STEPS:
-      NH_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64,
-               NH_ERR_ARG, "NH_SET_RING_STEPS: value %d not 0, 8, 16, 32 or 64",
+      NH_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64 ||
+                   value == 128,
+               NH_ERR_ARG, "NH_SET_RING_STEPS: value %d not 0, 8, 16, 32, 64 or 128",
                value);
       ctx->ring_steps = value;
       return NH_OK;

@@ -12,10 +12,11 @@
 // tridiagonal eigenvector, find_converged_eigval Method 2,
 // lanczos.c:764-996).  Eigenvectors come from a second pass over the same
 // recurrence (eigenvectorsLanczos, lanczos.c:1364-1689): the Lanczos
-// vectors land in a ring of K slots (64, or 32 when memory is short) and
-// every K iterations one kernel adds Ring * Z to the eigenvector block.  The second pass reuses
-// alpha and beta, so with the SK layout its H*v kernel also applies the
-// three-term step (no separate update kernel, no reductions).
+// vectors land in a ring (K = 128 steps, fewer when memory is short) and
+// every K iterations one kernel adds Ring * Z to the eigenvector block.
+// The second pass reuses alpha and beta, so with the SK layout its H*v
+// kernel also applies the three-term step (no separate update kernel, no
+// reductions), and the ring holds the r vectors themselves.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -30,7 +31,7 @@
 namespace nh {
 namespace {
 
-constexpr int kRingMax = 64;
+constexpr int kRingMax = 128;
 constexpr int kMaxVec = 64;
 
 __device__ __forceinline__ double wg_sum320(double v) {
@@ -1017,7 +1018,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_TRY(alloc((void **)&V, (size_t)ne * n_ * sizeof(double2)));
     NH_TRY(alloc((void **)&dZ, (size_t)kMaxVec * kRingMax * sizeof(double)));
     NH_HIP(hipMemsetAsync(V, 0, (size_t)ne * n_ * sizeof(double2), c->stream));
-    // ring: K = 64 steps per flush when the ring takes at most half the
+    // ring: K = 128 steps per flush when the ring takes at most half the
     // free memory (the run's buffers are released before any later stage
     // allocates), else 32, 16 or 8 (each flush reads and writes the
     // eigenvectors once)
