@@ -160,8 +160,10 @@ def log(msg):
 
 
 # convergence check (tridiagonal analysis) every CHECK_SKIP steps; the host
-# analysis overlaps the next block of device steps
-CHECK_SKIP = 50
+# analysis overlaps the next block of device steps, so the run goes one
+# block past the converged check point (measured: 25 saves ~30 ms per solve
+# against 50, with the same stopping points)
+CHECK_SKIP = 25
 
 
 def plain_solve(H, max_iter, windows=WINDOWS):
