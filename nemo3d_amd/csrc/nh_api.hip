@@ -121,8 +121,8 @@ static void destroy(nh_context *c) {
   delete c;
 }
 
-// Orders an atom list by 8x8x8-cell bricks (bricks in z, y, x order, cells
-// and atoms in their original order inside a brick).  The SK kernels give
+// Orders an atom list by 32x32x32-cell bricks (bricks in z, y, x order,
+// cells and atoms in their original order inside a brick).  The SK kernels give
 // consecutive workgroups consecutive list entries and each XCD a contiguous
 // part of the list, so a brick's atoms and most of their neighbours are in
 // flight together and the neighbours' vector entries are reused from the
@@ -154,7 +154,11 @@ static void brick_order(const nh_model *m, std::vector<int32_t> &list) {
   int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX};
   for (int i = 0; i < A; ++i)
     for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], pos[(size_t)i * 3 + d]);
-  constexpr int kBrick = 32;  // 8 cells of 4 units (a/4)
+  // 32 cells of 4 units (a/4).  Measured on the 786,520-atom dot (FP32 /
+  // FP64 fused Lanczos H*v): 4-cell bricks 0.241 / -, 8-cell 0.239 / 0.338,
+  // 16-cell 0.234 / 0.337, 32-cell 0.229-0.232 / 0.337-0.338 ms; octants
+  // at the medians (one per XCD) with 8-cell bricks inside 0.245 / 0.343
+  constexpr int kBrick = 128;
   auto key = [&](int32_t i) {
     int64_t b[3];
     for (int d = 0; d < 3; ++d) b[d] = (pos[(size_t)i * 3 + d] - lo[d]) / kBrick;
