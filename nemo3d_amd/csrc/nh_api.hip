@@ -156,8 +156,9 @@ static void brick_order(const nh_model *m, std::vector<int32_t> &list) {
     for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], pos[(size_t)i * 3 + d]);
   // 32 cells of 4 units (a/4).  Measured on the 786,520-atom dot (FP32 /
   // FP64 fused Lanczos H*v): 4-cell bricks 0.241 / -, 8-cell 0.239 / 0.338,
-  // 16-cell 0.234 / 0.337, 32-cell 0.229-0.232 / 0.337-0.338 ms; octants
-  // at the medians (one per XCD) with 8-cell bricks inside 0.245 / 0.343
+  // 16-cell 0.234 / 0.337, 32-cell 0.229-0.232 / 0.337-0.338, 64-cell
+  // 0.234 / 0.339 ms; octants at the medians (one per XCD) with 8-cell
+  // bricks inside 0.245 / 0.343
   constexpr int kBrick = 128;
   auto key = [&](int32_t i) {
     int64_t b[3];
