@@ -6,18 +6,32 @@
  * cpu_baseline leg of bench.py as the checker.  The product path
  * (nemo3d_amd/ and libnemohip.so) never links, loads or calls it.
  *
- * Parity status per function (how each one is pinned):
+ * Parity status per function (how each one is pinned; the tests named
+ * are in tests/test_oracle.py and tests/test_oracle_pins.py):
  *   or_sk_bond      - golden GaAs bulk 20-band spectrum (40 k-points,
  *                     nemo3d/example/TestRun/GaAs_bulk.old_nd_Ek_ascii),
- *                     the silicon4 Rayleigh quotients, and for strained
+ *                     the silicon4 Rayleigh quotient, and for strained
  *                     bonds the reference's own h3d_offdiag_diag_corr
- *                     compiled into oracle/_ref (when it builds).
- *   or_onsite       - same goldens; and the reference h3d_diag via _ref.
+ *                     compiled into oracle/_ref (fixture
+ *                     tests/golden/ref_sk_bonds.npz).
+ *   or_onsite       - the reference's own h3d_diag compiled into
+ *                     oracle/_ref (fixture tests/golden/ref_onsite.npz:
+ *                     GaAs/InAs records and random parameter vectors) and
+ *                     the GaAs bulk golden.
  *   or_assemble     - silicon4 (64,000-atom Si box, dangling-bond
- *                     passivation) Rayleigh quotients and residuals of the
- *                     shipped eigenvectors; GaAs bulk golden.
- *   or_hv           - dense-matrix brute force on small boxes.
- *   or_lanczos_*    - closed forms (2-D Laplacian spectrum), dense eigh.
+ *                     passivation) Rayleigh quotient and residual of the
+ *                     shipped eigenvector; GaAs bulk golden; the
+ *                     mixed-species onsite average by the linearity
+ *                     property (an anion's onsite record with k In and
+ *                     n-k Ga neighbours is the k/n mix of its pure-InAs and
+ *                     pure-GaAs records, interior and surface atoms).
+ *   or_hv           - dense-matrix brute force on small boxes (Hermiticity,
+ *                     the golden spectra above).
+ *   or_lanczos_tridiag - the 2-D Laplacian of NEMO_3D/eigen/doc/
+ *                     lanczos.tex:362-400: converged Ritz values equal the
+ *                     closed-form eigenvalues.
+ *   or_tridiag_eigs - LAPACK's tridiagonal eigensolver (scipy) on Lanczos
+ *                     and random tridiagonals, several windows.
  */
 #ifndef NEMO_ORACLE_H
 #define NEMO_ORACLE_H

@@ -149,6 +149,31 @@ def sk_bond(par, strn, a0, nnv, nnv0):
     return T, ca, cc
 
 
+def onsite(par, anion):
+    """or_onsite: spatial diagonal energies (10) and the spin-orbit
+    constant of an (averaged) parameter set; anion selects the anion
+    energies."""
+    par = np.ascontiguousarray(par, dtype=np.float64)
+    d = np.zeros(10)
+    lam = ctypes.c_double()
+    lib().or_onsite(_p(par), int(anion), _p(d), ctypes.byref(lam))
+    return d, lam.value
+
+
+def onsite_block(nb, diag, lam, sp=None):
+    """Dense nb x nb onsite block of one atom with the given onsite record
+    (diagonal, surface s-p couplings, spin-orbit constant), applied through
+    or_hv to unit vectors."""
+    rec = np.zeros((1, ONSITE_STRIDE))
+    rec[0, :10] = diag
+    if sp is not None:
+        rec[0, 10:16] = sp
+    rec[0, 16] = lam
+    H = OracleHamiltonian(nb, -np.ones((1, 4), dtype=np.int32), rec,
+                          np.zeros((1, 4, 10, 10)), 0)
+    return H.dense()
+
+
 def tridiag_eigs(d, e, lo, hi, tol=1e-15, wmax=100000):
     d = np.ascontiguousarray(d, dtype=np.float64)
     e = np.ascontiguousarray(e, dtype=np.float64)

@@ -3,9 +3,12 @@
 Restates calc_estrain (NEMO_3D/nemo3d/src/base/strain3d.c:1739-1890):
   E = sum_i [ sum_n a_in/(2 d0_in^2) (|r_in|^2 - d0_in^2)^2
             + sum_{n<p} sqrt(b_in b_ip)/(|d0_in||d0_ip|) (r_in.r_ip - d0_in.d0_ip)^2 ]
-Pinned by: zero energy and gradient for an ideal crystal at its own lattice
-constant, the closed-form energy of a uniformly scaled crystal, and central
-finite differences (the GPU gradient is checked against this energy).
+Pinned by: Keating's elastic constants C11 = (alpha + 3 beta)/a and
+C12 = (alpha - beta)/a (Keating 1966, Martin 1970), times the reference's
+8/3 prefactor convention, from the energy of a periodic crystal under
+hydrostatic, uniaxial and biaxial strain; zero energy and gradient for an
+ideal crystal at its own lattice constant; central finite differences
+(the GPU gradient is checked against this energy).
 """
 import numpy as np
 

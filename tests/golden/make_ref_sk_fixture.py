@@ -1,8 +1,10 @@
-"""Writes tests/golden/ref_sk_bonds.npz: strained Slater-Koster blocks and
-Boykin diagonal corrections computed by the reference's own bond routine
-(h3d_offdiag_diag_corr, built from the reference sources by
-oracle/ref_build.sh into oracle/_ref/ref_sk).  Run in a container that has
-the reference tree:  sh oracle/ref_build.sh && python tests/golden/make_ref_sk_fixture.py
+"""Writes tests/golden/ref_sk_bonds.npz and tests/golden/ref_onsite.npz:
+strained Slater-Koster blocks with their Boykin diagonal corrections, and
+onsite blocks (diagonal energies + spin-orbit), computed by the reference's
+own routines (h3d_offdiag_diag_corr and h3d_diag, built from the reference
+sources by oracle/ref_build.sh into oracle/_ref/ref_sk).  Run in a
+container that has the reference tree:
+  sh oracle/ref_build.sh && python tests/golden/make_ref_sk_fixture.py
 """
 import os
 import subprocess
@@ -50,7 +52,47 @@ def run(cs, harness=HARNESS):
     return ho, hd
 
 
+def onsite_cases(seed=1):
+    """Onsite parameter sets: the GaAs and InAs records, the mean of the
+    two (an As atom with Ga and In neighbours) and random vectors (every
+    entry distinct, so a swapped index shows), both band models, both
+    roles."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for band in (materials.BANDS_20_SPIN, materials.BANDS_10_NOSPIN):
+        t = materials.param_init(["GaAs", "InAs"], band)
+        nb = 20 if band == materials.BANDS_20_SPIN else 10
+        As = t.species.index("As")
+        ga = t.parmat[t.species.index("Ga")][As]
+        inn = t.parmat[t.species.index("In")][As]
+        pars = [ga, inn, 0.5 * (ga + inn)] + [rng.normal(0, 5, 35) for _ in range(3)]
+        for par in pars:
+            for anion in (0, 1):
+                out.append((nb, anion, np.array(par, float)))
+    return out
+
+
+def run_onsite(cs, harness=HARNESS):
+    lines = ["%d %d %s" % (nb, anion, " ".join("%.17g" % v for v in par))
+             for nb, anion, par in cs]
+    res = subprocess.run([harness, "diag"], input="\n".join(lines) + "\n",
+                         capture_output=True, text=True, check=True)
+    rows = [np.array(l.split(), float) for l in res.stdout.strip().split("\n")]
+    h = np.zeros((len(rows), 20, 20), dtype=np.complex128)
+    for k, (r, (nb, _, _)) in enumerate(zip(rows, cs)):
+        v = r.reshape(nb, nb, 2)
+        h[k, :nb, :nb] = v[..., 0] + 1j * v[..., 1]
+    return h
+
+
 if __name__ == "__main__":
+    oc = onsite_cases()
+    h = run_onsite(oc)
+    np.savez_compressed(
+        os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_onsite.npz"),
+        nb=np.array([c[0] for c in oc]), anion=np.array([c[1] for c in oc]),
+        par=np.array([c[2] for c in oc]), h=h)
+    print("wrote", len(oc), "onsite blocks")
     cs = cases()
     ho, hd = run(cs)
     np.savez_compressed(

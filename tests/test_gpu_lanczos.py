@@ -240,7 +240,7 @@ def test_fp32_lanczos_vectors_at_fp32_level():
                                                ("blocks", "fp64")])
 def test_eigenvector_ring_length(layout, precision):
     """The eigenvector pass adds its ring into the eigenvectors every K
-    steps (K = 64, fewer when HBM is short): K = 8 gives the same
+    steps (K = 128, fewer when HBM is short): K = 8 gives the same
     eigenpairs to rounding (SK: rings of r, FP64 or FP32 copies; stored
     blocks: ring of q)."""
     M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=24)

@@ -208,3 +208,94 @@ def test_strained_bonds_match_reference_harness(tmp_path):
              strain=np.array([c[6] for c in cs]), ho=ho, hd=hd)
     worst_t, worst_c = _ref_sk_check(d)
     assert worst_t < 1e-13 and worst_c < 1e-14, (worst_t, worst_c)
+
+
+def _ref_onsite_worst(data):
+    """Oracle onsite (or_onsite, applied through or_hv) against the
+    reference h3d_diag blocks of the fixture."""
+    worst = 0.0
+    for k in range(len(data["nb"])):
+        nb = int(data["nb"][k])
+        d, lam = oracle.onsite(data["par"][k], data["anion"][k])
+        blk = oracle.onsite_block(nb, d, lam)
+        worst = max(worst, np.abs(blk - data["h"][k][:nb, :nb]).max())
+    return worst
+
+
+def test_onsite_matches_reference_fixture():
+    """Onsite blocks (diagonal energies and the spin-orbit entries) of the
+    GaAs/InAs records, their mean and random parameter vectors, as written
+    by the reference's own h3d_diag (mb_ham_spds_micro.c:1453-1505), both
+    band models and both roles."""
+    d = np.load(os.path.join(GOLDEN, "ref_onsite.npz"))
+    assert np.abs(d["h"].imag).max() > 1e-3  # spin-orbit exercised
+    assert _ref_onsite_worst(d) < 1e-14
+
+
+def test_onsite_fixture_detects_planted_error():
+    """Swapping two parameter entries in the oracle's input (the s and s*
+    energies of the anion) breaks the agreement."""
+    d = dict(np.load(os.path.join(GOLDEN, "ref_onsite.npz")))
+    d["par"] = d["par"].copy()
+    d["par"][:, [0, 1]] = d["par"][:, [1, 0]]
+    assert _ref_onsite_worst(d) > 1e-3
+
+
+def test_onsite_matches_reference_harness():
+    """Fresh random parameter vectors through the compiled h3d_diag."""
+    harness = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "ref_sk")
+    if not os.path.exists(harness):
+        pytest.skip("reference harness not built (no reference tree)")
+    sys.path.insert(0, GOLDEN)
+    import make_ref_sk_fixture as mk
+    cs = mk.onsite_cases(seed=12)
+    h = mk.run_onsite(cs, harness)
+    d = dict(nb=np.array([c[0] for c in cs]), anion=np.array([c[1] for c in cs]),
+             par=np.array([c[2] for c in cs]), h=h)
+    assert _ref_onsite_worst(d) < 1e-14
+
+
+@pytest.mark.parametrize("band", ["spin", "nospin"])
+def test_mixed_species_onsite_is_the_neighbour_mean(band):
+    """Onsite terms of an anion with cations of two species around it.
+    H_Diag_Wrapper (h_cvectr_mult.c:854-918) averages parmat[c][a] and the
+    strain shift over the present neighbours and h3d_diag is linear in the
+    parameters (pinned above against the reference routine), so the
+    onsite block of an As atom with k In and (n - k) Ga neighbours is the
+    k/n : (n-k)/n mix of its blocks in pure InAs and pure GaAs, also at a
+    (001) surface with n = 2.  Taking one neighbour's record, or dividing
+    by 4 at a surface, breaks this."""
+    bm = materials.BANDS_20_SPIN if band == "spin" else materials.BANDS_10_NOSPIN
+    a = 0.56532
+    s = structure.build((0, 0, 0), (3, 3, 3), lambda pos, m, cell: np.where(
+        m < 4, 1, 2), ["In", "Ga", "As"], (a, a, a))
+    rng = np.random.default_rng(5)
+    s.species = s.species.copy()
+    cat = np.nonzero(s.is_cation)[0]
+    s.species[cat] = rng.choice([0, 1], cat.size)  # In or Ga per cation site
+
+    def arrays(species_of_cations=None):
+        s2 = structure.Structure(s.elements, s.species.copy(), s.is_cation, s.nbr,
+                                 s.bond_q, s.disp, s.cell, s.m, s.lattice,
+                                 s.periodic, s.wrapped)
+        if species_of_cations is not None:
+            s2.species[cat] = species_of_cations
+        return model.TBModel(s2, ["InAs", "GaAs"], bm).arrays()
+
+    H = oracle.assemble(arrays())
+    Hin = oracle.assemble(arrays(0))
+    Hga = oracle.assemble(arrays(1))
+    checked = set()
+    for i in np.nonzero(~s.is_cation.astype(bool))[0]:
+        nb_i = s.nbr[i][s.nbr[i] >= 0]
+        n = nb_i.size
+        k = int((s.species[nb_i] == 0).sum())
+        if n < 2:
+            continue
+        want = (k * Hin.onsite[i] + (n - k) * Hga.onsite[i]) / n
+        # surface s-p couplings and passivation do not depend on species
+        assert np.abs(H.onsite[i] - want).max() < 1e-12, (i, n, k)
+        checked.add((n, k))
+    # interior and surface anions with mixed neighbourhoods were seen
+    assert {(4, 1), (4, 2), (4, 3)} <= checked
+    assert any(n < 4 and 0 < k < n for n, k in checked)
