@@ -130,7 +130,15 @@ nh_status nh_ham_export(nh_context *ctx, double *onsite, double *blocks);
 
 /* ---- device buffers -------------------------------------------------------
  * The library owns its device memory (it uses the system ROCm runtime, so
- * it does not share allocations with other HIP runtimes in the process). */
+ * it does not share allocations with other HIP runtimes in the process).
+ * Solver workspace is kept in a per-context pool between solves (no
+ * reference counterpart: the reference allocates host memory per solve);
+ * every allocation of the library, nh_device_alloc included, first hands
+ * the idle pool blocks of all contexts on the device back when the device
+ * is full.  nh_context_mem_info reports the device's free bytes and the
+ * idle pool bytes of all contexts on it (either pointer may be NULL). */
+nh_status nh_context_mem_info(nh_context *ctx, int64_t *free_bytes,
+                              int64_t *pool_idle_bytes);
 nh_status nh_device_alloc(nh_context *ctx, int64_t bytes, void **dptr);
 nh_status nh_device_free(nh_context *ctx, void *dptr);
 nh_status nh_memcpy_h2d(nh_context *ctx, void *dst, const void *src,

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "nh_internal.h"
@@ -12,6 +13,12 @@
 namespace nh {
 
 static thread_local char g_err[1024] = "";
+
+// Every live context, so that an allocation that fails on a device can take
+// back the idle workspace blocks of all contexts on that device, and ring
+// sizing can count them as available.
+static std::mutex g_ctx_mu;
+static std::vector<nh_context *> g_ctxs;
 
 Progress::Progress() {
   const char *e = std::getenv("NEMOHIP_PROGRESS");
@@ -37,6 +44,43 @@ void set_error(const char *fmt, ...) {
   va_end(ap);
 }
 
+// Frees the idle workspace blocks of one context (after its queued work).
+static nh_status pool_trim(nh_context *c) {
+  NH_HIP(hipStreamSynchronize(c->stream));
+  std::vector<nh_context::PoolBlock> keep;
+  for (auto &b : c->pool) {
+    if (b.used) keep.push_back(b);
+    else NH_HIP(hipFree(b.p));
+  }
+  c->pool.swap(keep);
+  return NH_OK;
+}
+
+nh_status device_malloc(nh_context *c, void **p, size_t bytes) {
+  bytes = bytes > 0 ? bytes : 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    // give the idle workspace of every context on this device back and
+    // try once more
+    (void)hipGetLastError();
+    std::vector<nh_context *> same;
+    {
+      std::lock_guard<std::mutex> g(g_ctx_mu);
+      for (nh_context *o : g_ctxs)
+        if (o->device == c->device) same.push_back(o);
+    }
+    for (nh_context *o : same) NH_TRY(pool_trim(o));
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    set_error("hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return NH_ERR_NOMEM;
+  }
+  return NH_OK;
+}
+
 nh_status pool_alloc(nh_context *c, void **p, size_t bytes) {
   bytes = bytes > 0 ? (bytes + 255) / 256 * 256 : 256;
   int best = -1;
@@ -51,21 +95,7 @@ nh_status pool_alloc(nh_context *c, void **p, size_t bytes) {
     *p = c->pool[best].p;
     return NH_OK;
   }
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) {
-    // return the free blocks to the device and try once more
-    (void)hipGetLastError();
-    NH_HIP(hipStreamSynchronize(c->stream));
-    std::vector<nh_context::PoolBlock> keep;
-    for (auto &b : c->pool) {
-      if (b.used) keep.push_back(b);
-      else NH_HIP(hipFree(b.p));
-    }
-    c->pool.swap(keep);
-    e = hipMalloc(p, bytes);
-  }
-  NH_CHECK(e == hipSuccess, NH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes,
-           hipGetErrorString(e));
+  NH_TRY(device_malloc(c, p, bytes));
   c->pool.push_back({*p, bytes, true});
   return NH_OK;
 }
@@ -79,20 +109,24 @@ void pool_release(nh_context *c, void *p) {
 }
 
 size_t pool_free_bytes(const nh_context *c) {
+  std::lock_guard<std::mutex> g(g_ctx_mu);
   size_t s = 0;
-  for (const auto &b : c->pool)
-    if (!b.used) s += b.bytes;
+  for (const nh_context *o : g_ctxs)
+    if (o->device == c->device)
+      for (const auto &b : o->pool)
+        if (!b.used) s += b.bytes;
   return s;
 }
 
 nh_status ensure_buffers(nh_context *c, int64_t elems) {
   if (elems <= c->buf_cap) return NH_OK;
+  NH_HIP(hipStreamSynchronize(c->stream));
   hipFree(c->xbuf);
   hipFree(c->ybuf);
   c->xbuf = c->ybuf = nullptr;
   c->buf_cap = 0;
-  NH_HIP(hipMalloc(&c->xbuf, elems * sizeof(nh_cplx)));
-  NH_HIP(hipMalloc(&c->ybuf, elems * sizeof(nh_cplx)));
+  NH_TRY(device_malloc(c, (void **)&c->xbuf, elems * sizeof(nh_cplx)));
+  NH_TRY(device_malloc(c, (void **)&c->ybuf, elems * sizeof(nh_cplx)));
   c->buf_cap = elems;
   return NH_OK;
 }
@@ -106,6 +140,10 @@ static nh_status upload(T **dst, const T *src, size_t count) {
 
 static void destroy(nh_context *c) {
   if (!c) return;
+  {
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    g_ctxs.erase(std::remove(g_ctxs.begin(), g_ctxs.end(), c), g_ctxs.end());
+  }
   hipSetDevice(c->device);
   void *ptrs[] = {c->parmat, c->strnmat, c->species, c->is_cation, c->nbr,
                   c->bond_q, c->disp, c->phi, c->cations, c->anions, c->bond_of,
@@ -255,6 +293,10 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
     destroy(c);
     return st;
   }
+  {
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    g_ctxs.push_back(c);
+  }
   *out = c;
   return NH_OK;
 }
@@ -361,13 +403,20 @@ nh_status nh_ham_export(nh_context *ctx, double *onsite, double *blocks) {
   return NH_OK;
 }
 
+nh_status nh_context_mem_info(nh_context *ctx, int64_t *free_bytes,
+                              int64_t *pool_idle_bytes) {
+  NH_TRY(check_ctx(ctx));
+  size_t fr = 0, tot = 0;
+  NH_HIP(hipMemGetInfo(&fr, &tot));
+  if (free_bytes) *free_bytes = (int64_t)fr;
+  if (pool_idle_bytes) *pool_idle_bytes = (int64_t)pool_free_bytes(ctx);
+  return NH_OK;
+}
+
 nh_status nh_device_alloc(nh_context *ctx, int64_t bytes, void **dptr) {
   NH_TRY(check_ctx(ctx));
   NH_CHECK(dptr && bytes >= 0, NH_ERR_ARG, "nh_device_alloc: bad arguments");
-  hipError_t e = hipMalloc(dptr, bytes > 0 ? bytes : 16);
-  NH_CHECK(e == hipSuccess, NH_ERR_NOMEM, "hipMalloc(%lld): %s",
-           (long long)bytes, hipGetErrorString(e));
-  return NH_OK;
+  return device_malloc(ctx, dptr, (size_t)bytes);
 }
 
 nh_status nh_device_free(nh_context *ctx, void *dptr) {

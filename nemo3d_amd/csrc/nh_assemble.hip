@@ -242,14 +242,14 @@ nh_status assemble(nh_context *c, int layout, int precision) {
   free_ham(c);
   const int A = c->n_atoms;
   const size_t nblk = (size_t)A * 4 * 100;
-  NH_HIP(hipMalloc(&c->onsite, (size_t)A * kOnsiteStride * sizeof(double)));
-  NH_HIP(hipMalloc(&c->blocks64, nblk * sizeof(double)));
-  NH_HIP(hipMalloc(&c->corr, (size_t)A * 4 * 10 * sizeof(double)));
+  NH_TRY(device_malloc(c, (void **)&c->onsite, (size_t)A * kOnsiteStride * sizeof(double)));
+  NH_TRY(device_malloc(c, (void **)&c->blocks64, nblk * sizeof(double)));
+  NH_TRY(device_malloc(c, (void **)&c->corr, (size_t)A * 4 * 10 * sizeof(double)));
   NH_HIP(hipMemsetAsync(c->blocks64, 0, nblk * sizeof(double), c->stream));
   NH_HIP(hipMemsetAsync(c->corr, 0, (size_t)A * 40 * sizeof(double), c->stream));
   double *sk = nullptr;
   if (layout == NH_LAYOUT_SK) {
-    NH_HIP(hipMalloc(&c->sk64, (size_t)c->n_cations * 4 * kSkStride * sizeof(double)));
+    NH_TRY(device_malloc(c, (void **)&c->sk64, (size_t)c->n_cations * 4 * kSkStride * sizeof(double)));
     NH_HIP(hipMemsetAsync(c->sk64, 0,
                           (size_t)c->n_cations * 4 * kSkStride * sizeof(double),
                           c->stream));
@@ -268,16 +268,16 @@ nh_status assemble(nh_context *c, int layout, int precision) {
                      c->corr, c->onsite);
   NH_HIP(hipGetLastError());
   if (precision == NH_PREC_FP32) {
-    NH_HIP(hipMalloc(&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
+    NH_TRY(device_malloc(c, (void **)&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
     hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
                        (int64_t)A * kOnsiteStride, c->onsite, c->onsite32);
     if (layout == NH_LAYOUT_BLOCKS) {
-      NH_HIP(hipMalloc(&c->blocks32, nblk * sizeof(float)));
+      NH_TRY(device_malloc(c, (void **)&c->blocks32, nblk * sizeof(float)));
       hipLaunchKernelGGL(k_convert<float>, dim3(4096), dim3(256), 0, c->stream,
                          (int64_t)nblk, c->blocks64, c->blocks32);
     } else {
       const int64_t ns = (int64_t)c->n_cations * 4 * kSkStride;
-      NH_HIP(hipMalloc(&c->sk32, ns * sizeof(float)));
+      NH_TRY(device_malloc(c, (void **)&c->sk32, ns * sizeof(float)));
       hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
                          ns, c->sk64, c->sk32);
     }
@@ -298,19 +298,19 @@ nh_status select_precision(nh_context *c, int precision) {
   if (precision == NH_PREC_FP32) {
     // the FP64 values are always kept: derive the FP32 copies once
     if (!c->onsite32) {
-      NH_HIP(hipMalloc(&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
+      NH_TRY(device_malloc(c, (void **)&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
       hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
                          (int64_t)A * kOnsiteStride, c->onsite, c->onsite32);
     }
     if (c->layout == NH_LAYOUT_BLOCKS && !c->blocks32) {
       const size_t nblk = (size_t)A * 4 * 100;
-      NH_HIP(hipMalloc(&c->blocks32, nblk * sizeof(float)));
+      NH_TRY(device_malloc(c, (void **)&c->blocks32, nblk * sizeof(float)));
       hipLaunchKernelGGL(k_convert<float>, dim3(4096), dim3(256), 0, c->stream,
                          (int64_t)nblk, c->blocks64, c->blocks32);
     }
     if (c->layout == NH_LAYOUT_SK && !c->sk32) {
       const int64_t ns = (int64_t)c->n_cations * 4 * kSkStride;
-      NH_HIP(hipMalloc(&c->sk32, ns * sizeof(float)));
+      NH_TRY(device_malloc(c, (void **)&c->sk32, ns * sizeof(float)));
       hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
                          ns, c->sk64, c->sk32);
     }

@@ -43,7 +43,7 @@ class BlockLanczos {
 
  private:
   nh_status alloc(void **q, size_t bytes) {
-    NH_HIP(hipMalloc(q, bytes));
+    NH_TRY(device_malloc(c_, (void **)q, bytes));
     mem_.push_back(*q);
     return NH_OK;
   }

@@ -320,14 +320,14 @@ nh_status ensure_acts(nh_cnn *c, int64_t P) {
   c->pre.assign(c->layers, nullptr);
   c->act.assign(c->layers, nullptr);
   for (int l = 0; l < c->layers; ++l) {
-    NH_HIP(hipMalloc(&c->pre[l], P * c->width * sizeof(float)));
-    NH_HIP(hipMalloc(&c->act[l], P * c->width * sizeof(float)));
+    NH_TRY(device_malloc(c->ctx, (void **)&c->pre[l], P * c->width * sizeof(float)));
+    NH_TRY(device_malloc(c->ctx, (void **)&c->act[l], P * c->width * sizeof(float)));
   }
   const int cmax = std::max(std::max(c->c_in, c->width), c->c_out);
-  NH_HIP(hipMalloc(&c->in, P * c->c_in * sizeof(float)));
-  NH_HIP(hipMalloc(&c->out, P * c->c_out * sizeof(float)));
-  NH_HIP(hipMalloc(&c->dact, P * cmax * sizeof(float)));
-  NH_HIP(hipMalloc(&c->dtmp, P * cmax * sizeof(float)));
+  NH_TRY(device_malloc(c->ctx, (void **)&c->in, P * c->c_in * sizeof(float)));
+  NH_TRY(device_malloc(c->ctx, (void **)&c->out, P * c->c_out * sizeof(float)));
+  NH_TRY(device_malloc(c->ctx, (void **)&c->dact, P * cmax * sizeof(float)));
+  NH_TRY(device_malloc(c->ctx, (void **)&c->dtmp, P * cmax * sizeof(float)));
   c->cap_points = P;
   return NH_OK;
 }
@@ -461,7 +461,7 @@ nh_status nh_cnn_create(nh_context *ctx, int c_in, int width, int layers,
   }
   nh_status st = NH_OK;
   auto mk = [&](void **p, size_t bytes) {
-    if (st == NH_OK && hipMalloc(p, bytes) != hipSuccess) {
+    if (st == NH_OK && device_malloc(c->ctx, p, bytes) != NH_OK) {
       set_error("nh_cnn_create: out of device memory");
       st = NH_ERR_NOMEM;
     }
@@ -568,15 +568,14 @@ nh_status nh_cnn_start_block(nh_cnn *c, const float *feat, int gx, int gy,
   NH_TRY(upload_feats(c, feat, P));
   NH_TRY(forward(c, gx, gy, gz));
   int32_t *dp = nullptr, *ds = nullptr;
-  NH_HIP(hipMalloc(&dp, A * sizeof(int32_t)));
-  NH_HIP(hipMalloc(&ds, A * sizeof(int32_t)));
+  NH_TRY(device_malloc(x, (void **)&dp, A * sizeof(int32_t)));
+  NH_TRY(device_malloc(x, (void **)&ds, A * sizeof(int32_t)));
   hipMemcpy(dp, atom_point, A * sizeof(int32_t), hipMemcpyHostToDevice);
   hipMemcpy(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice);
   double2 *Y = reinterpret_cast<double2 *>(block);
   double2 *tmp = nullptr;
   if (ptr_kind == NH_PTR_HOST) {
-    hipError_t e = hipMalloc(&tmp, (size_t)n_states * x->n * sizeof(double2));
-    if (e != hipSuccess) {
+    if (device_malloc(x, (void **)&tmp, (size_t)n_states * x->n * sizeof(double2)) != NH_OK) {
       hipFree(dp);
       hipFree(ds);
       NH_CHECK(false, NH_ERR_NOMEM, "nh_cnn_start_block: out of memory");
@@ -618,7 +617,7 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
   // 16 columns), G (gradient, 8 columns)
   std::vector<void *> mem;
   auto mk = [&](void **p, size_t bytes) -> nh_status {
-    NH_HIP(hipMalloc(p, bytes));
+    NH_TRY(device_malloc(x, p, bytes));
     mem.push_back(*p);
     return NH_OK;
   };

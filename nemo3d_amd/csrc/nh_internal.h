@@ -193,6 +193,14 @@ nh_status reduce_nrm2sq(nh_context *c, int64_t n, const nh_cplx *x,
 nh_status launch_axpby(nh_context *c, int64_t n, nh_cplx a, const nh_cplx *x,
                        nh_cplx b, nh_cplx *y);
 nh_status ensure_reduce(nh_context *c, int blocks);
+// hipMalloc that, when the device is full, frees the idle workspace blocks
+// of every context on the device and tries once more (NH_ERR_NOMEM after
+// that); every device allocation of the library goes through it
+nh_status device_malloc(nh_context *c, void **p, size_t bytes);
+template <typename T>
+inline nh_status device_alloc(nh_context *c, T **p, size_t count) {
+  return device_malloc(c, reinterpret_cast<void **>(p), count * sizeof(T));
+}
 // device memory from the context's workspace pool: the smallest free block
 // of at least `bytes` and at most twice that, else a new allocation (after
 // returning the free blocks to the device if it is full); pool_release
@@ -200,6 +208,7 @@ nh_status ensure_reduce(nh_context *c, int blocks);
 // still be running: the next user is ordered after it on that stream)
 nh_status pool_alloc(nh_context *c, void **p, size_t bytes);
 void pool_release(nh_context *c, void *p);
-size_t pool_free_bytes(const nh_context *c);  // bytes in free blocks
+// bytes in the idle blocks of every context on c's device
+size_t pool_free_bytes(const nh_context *c);
 nh_status ensure_buffers(nh_context *c, int64_t elems);
 }  // namespace nh

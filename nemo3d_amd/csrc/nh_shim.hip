@@ -99,7 +99,7 @@ nh_status nh_shim_set_wavefunctions(int nwf, const double *energy,
   g_shim.energy.assign(energy, energy + nwf);
   if (nwf == 0) return NH_OK;
   const size_t bytes = (size_t)nwf * c->n * sizeof(nh_cplx);
-  NH_HIP(hipMalloc(&g_shim.wf, bytes));
+  NH_TRY(nh::device_malloc(c, (void **)&g_shim.wf, bytes));
   NH_HIP(hipMemcpy(g_shim.wf, psi, bytes, hipMemcpyHostToDevice));
   g_shim.n = c->n;
   return NH_OK;

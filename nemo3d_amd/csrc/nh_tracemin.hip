@@ -34,7 +34,7 @@ class Tracemin {
   nh_status alloc(double2 **q, int cols) {
     const size_t bytes = (size_t)cols * (size_t)n_ * sizeof(double2);
     void *p = nullptr;
-    NH_HIP(hipMalloc(&p, bytes));
+    NH_TRY(device_malloc(c_, (void **)&p, bytes));
     NH_CHECK(p, NH_ERR_NOMEM, "TRACEMIN: allocation of %zu bytes failed", bytes);
     mem_.push_back(p);
     *q = static_cast<double2 *>(p);
@@ -170,13 +170,13 @@ nh_status Tracemin::run(const nh_tracemin_opts *o, nh_eig_result *res) {
   NH_TRY(alloc(&Qc, s_));
   NH_TRY(alloc(&tmp, s_));
   NH_TRY(alloc(&T1_, s_));
-  NH_HIP(hipMalloc(&part_, (size_t)kParts * 2 * kMaxP * kMaxP * sizeof(double)));
+  NH_TRY(device_malloc(c_, (void **)&part_, (size_t)kParts * 2 * kMaxP * kMaxP * sizeof(double)));
   mem_.push_back(part_);
-  NH_HIP(hipMalloc(&sum_, 2 * kMaxP * kMaxP * sizeof(double)));
+  NH_TRY(device_malloc(c_, (void **)&sum_, 2 * kMaxP * kMaxP * sizeof(double)));
   mem_.push_back(sum_);
-  NH_HIP(hipMalloc(&dM_, 2 * kMaxP * kMaxP * sizeof(double)));
+  NH_TRY(device_malloc(c_, (void **)&dM_, 2 * kMaxP * kMaxP * sizeof(double)));
   mem_.push_back(dM_);
-  NH_HIP(hipMalloc(&dAB_, 2 * kMaxP * sizeof(double)));
+  NH_TRY(device_malloc(c_, (void **)&dAB_, 2 * kMaxP * sizeof(double)));
   mem_.push_back(dAB_);
 
   // start block: caller's, or uniform (0,1) real/imag parts (zlarnv idist=1,

@@ -13,7 +13,7 @@ nh_status ensure_reduce(nh_context *c, int blocks) {
   c->red = nullptr;
   c->hred = nullptr;
   const int cap = blocks < 4096 ? 4096 : blocks;
-  NH_HIP(hipMalloc(&c->red, (size_t)(cap + 64) * 2 * sizeof(double)));
+  NH_TRY(device_malloc(c, (void **)&c->red, (size_t)(cap + 64) * 2 * sizeof(double)));
   NH_HIP(hipHostMalloc(&c->hred, 64 * sizeof(double)));
   c->red_cap = cap;
   return NH_OK;

@@ -161,7 +161,7 @@ class Vff {
 
  private:
   nh_status alloc(double **p, size_t n) {
-    NH_HIP(hipMalloc(p, n * sizeof(double)));
+    NH_TRY(device_malloc(c_, (void **)p, n * sizeof(double)));
     mem_.push_back(*p);
     NH_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), c_->stream));
     return NH_OK;
@@ -297,7 +297,7 @@ extern "C" nh_status nh_vff_energy_grad(nh_context *ctx, const double *alpha,
   const size_t sz[6] = {(size_t)S * S, (size_t)S * S, (size_t)S * S,
                         (size_t)A * 3, (size_t)A * 3,
                         (size_t)(A / nh::kRedThreads + 2) * 2 + 8};
-  for (int k = 0; k < 6; ++k) NH_HIP(hipMalloc(&m[k], sz[k] * sizeof(double)));
+  for (int k = 0; k < 6; ++k) NH_TRY(nh::device_malloc(ctx, (void **)&m[k], sz[k] * sizeof(double)));
   hipMemcpy(m[0], alpha, sz[0] * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(m[1], beta, sz[1] * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(m[2], a0, sz[2] * sizeof(double), hipMemcpyHostToDevice);
@@ -329,9 +329,9 @@ extern "C" nh_status nh_vff_relax(nh_context *ctx, const double *alpha,
   NH_HIP(hipSetDevice(ctx->device));
   const int S = ctx->n_species;
   double *dal = nullptr, *dbe = nullptr, *da0 = nullptr;
-  NH_HIP(hipMalloc(&dal, S * S * sizeof(double)));
-  NH_HIP(hipMalloc(&dbe, S * S * sizeof(double)));
-  NH_HIP(hipMalloc(&da0, S * S * sizeof(double)));
+  NH_TRY(nh::device_malloc(ctx, (void **)&dal, S * S * sizeof(double)));
+  NH_TRY(nh::device_malloc(ctx, (void **)&dbe, S * S * sizeof(double)));
+  NH_TRY(nh::device_malloc(ctx, (void **)&da0, S * S * sizeof(double)));
   hipMemcpy(dal, alpha, S * S * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(dbe, beta, S * S * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(da0, a0, S * S * sizeof(double), hipMemcpyHostToDevice);

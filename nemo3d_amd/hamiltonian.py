@@ -7,6 +7,7 @@ may be numpy arrays (host, copied) or native.DeviceArray buffers (device,
 used in place).
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -17,6 +18,7 @@ class Hamiltonian:
     def __init__(self, tbmodel, device=0, layout=native.LAYOUT_BLOCKS,
                  precision=native.PREC_FP64):
         self.model = tbmodel
+        self._device_arrays = weakref.WeakSet()
         lib = native.load()
         arr = tbmodel.arrays()
         self._keep = {}
@@ -52,6 +54,9 @@ class Hamiltonian:
 
     def close(self):
         if self._ctx:
+            # device arrays made from this context are freed first
+            for a in list(self._device_arrays):
+                a.free()
             native.load().nh_context_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
 

@@ -158,6 +158,7 @@ SIGNATURES = {
     "nh_hV": (ctypes.c_int, [P, ctypes.c_double, P, P, ctypes.c_int,
                              ctypes.c_int64, ctypes.c_int]),
     "nh_ham_export": (ctypes.c_int, [P, P, P]),
+    "nh_context_mem_info": (ctypes.c_int, [P, P, P]),
     "nh_device_alloc": (ctypes.c_int, [P, ctypes.c_int64, P]),
     "nh_device_free": (ctypes.c_int, [P, P]),
     "nh_memcpy_h2d": (ctypes.c_int, [P, P, P, ctypes.c_int64]),
@@ -240,17 +241,29 @@ class DeviceArray:
     """Device allocation made through a libnemohip context.
 
     `owner` is the object holding the context (it must expose `.ctx`); the
-    array keeps a reference to it so the context outlives the buffer."""
+    array keeps a reference to it so the context outlives the buffer.  An
+    owner with a `_device_arrays` set (Hamiltonian) frees its live arrays
+    before it destroys the context; an array whose owner's context is gone
+    is not freed again."""
 
     def __init__(self, owner, shape, dtype=np.complex128):
         self.owner = owner
         ctx = owner.ctx
-        self.ctx = ctx
         self.shape = tuple(np.atleast_1d(shape))
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         self.ptr = ctypes.c_void_p()
         check(load().nh_device_alloc(ctx, self.nbytes, ctypes.byref(self.ptr)))
+        reg = getattr(owner, "_device_arrays", None)
+        if reg is not None:
+            reg.add(self)
+
+    @property
+    def ctx(self):
+        ctx = self.owner.ctx
+        if not ctx:
+            raise RuntimeError("DeviceArray: its context has been destroyed")
+        return ctx
 
     @classmethod
     def from_host(cls, owner, a):
@@ -271,7 +284,9 @@ class DeviceArray:
 
     def free(self):
         if self.ptr:
-            check(load().nh_device_free(self.ctx, self.ptr))
+            ctx = self.owner.ctx
+            if ctx:
+                check(load().nh_device_free(ctx, self.ptr))
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):
