@@ -82,12 +82,6 @@ HNORM = WORKLOADS["dot786k"]["hnorm"]
 WINDOWS = WORKLOADS["dot786k"]["windows"]
 
 
-def build_dot(workload="dot786k"):
-    M = WORKLOADS[workload]["build"]()
-    info = vff.relax(M)
-    return M, info
-
-
 def lanczos_memory(n, free_bytes=288e9, k=8):
     """Device bytes of the Lanczos vectors at N = n (SK layout): the
     eigenvalue pass keeps r_k, r_(k-1) and y in FP64 (+ two FP32 copies of r
@@ -127,24 +121,67 @@ def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False):
     return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
-def cpu_baseline(M, matvecs, sample_iters=2):
-    """The oracle (C restatement, 1 core) on a bounded sample of the plain
-    solve: assembly plus `sample_iters` Lanczos iterations of the same
-    Hamiltonian, scaled to the plain solve's H·v count."""
+def host_threads():
+    """Threads the host offers this process (OMP_NUM_THREADS when set, else
+    the CPU affinity)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(M, matvecs, checks, sample_iters=2):
+    """The oracle (C restatement of the reference's assembly, H*v and
+    Lanczos recurrence) on the host: assembly plus `sample_iters` Lanczos
+    iterations of the same Hamiltonian, on 1 thread and on every host
+    thread, scaled to the plain solve's H*v count; and, as the checker,
+    the FP64 residual and Rayleigh quotient of each GPU eigenpair in
+    `checks` ({form: (eigenvalues, vectors)}) under the oracle's operator."""
     from oracle import oracle
+    nthr = host_threads()
+    oracle.set_threads(nthr)
     t0 = time.time()
     Ho = oracle.assemble(M.arrays())
     t_asm = time.time() - t0
     r0 = np.ones(M.n, dtype=np.complex128) / np.sqrt(M.n)
-    t0 = time.time()
-    Ho.lanczos_tridiag(r0, sample_iters)
-    per_it = (time.time() - t0) / sample_iters
-    return {"value": per_it * matvecs + t_asm, "unit": "s", "cores": 1,
+    per_it = {}
+    for th in (1, nthr):
+        oracle.set_threads(th)
+        t0 = time.time()
+        Ho.lanczos_tridiag(r0, sample_iters)
+        per_it[th] = (time.time() - t0) / sample_iters
+    oracle.set_threads(nthr)
+    base = {"value": per_it[nthr] * matvecs + t_asm, "unit": "s", "cores": nthr,
             "kind": "port",
-            "sample": "oracle C restatement, 1 thread: assembly (%.1f s) + %d "
-                      "Lanczos iterations timed (%.2f s/iteration), scaled to "
-                      "the plain GPU solve's %d H*v" % (t_asm, sample_iters,
-                                                       per_it, matvecs)}
+            "value_1_thread": per_it[1] * matvecs + t_asm,
+            "host": "%s, %d threads used of %d CPUs" % (cpu_model(), nthr,
+                                                       os.cpu_count()),
+            "sample": "oracle C restatement (OpenMP over atoms): assembly "
+                      "(%.1f s, %d threads) + %d Lanczos iterations timed on %d "
+                      "threads (%.3f s/iteration) and on 1 thread (%.3f "
+                      "s/iteration), scaled to the plain GPU solve's %d H*v"
+                      % (t_asm, nthr, sample_iters, nthr, per_it[nthr],
+                         per_it[1], matvecs)}
+    check = {}
+    for form, (ev, vecs) in checks.items():
+        res, dev = [], []
+        for e, v in zip(ev, vecs):
+            hv = Ho.hv(v)
+            nv = np.vdot(v, v).real
+            res.append(float(np.linalg.norm(hv - e * v) / np.sqrt(nv)))
+            dev.append(float(np.vdot(v, hv).real / nv - e))
+        check[form] = {"residual_eV": res, "rayleigh_minus_E_eV": dev}
+    return base, check
 
 
 def traffic_from_profiles(name):
@@ -166,10 +203,10 @@ def log(msg):
 CHECK_SKIP = 25
 
 
-def plain_solve(H, max_iter, windows=WINDOWS):
+def plain_solve(H, max_iter, windows=WINDOWS, vectors_out=None):
     return eigensolve_lanczos(H, num_vb=4, num_cb=4, max_iter=max_iter, tol=1e-8,
                               start_iter=200, skip=CHECK_SKIP, stop_rule="window",
-                              **windows)
+                              capacity=8, vectors_out=vectors_out, **windows)
 
 
 # the FP32 stage stops at 3e-8 (its vectors cannot go below the FP32
@@ -186,7 +223,8 @@ def mixed_solve(H, start, max_iter, windows=WINDOWS, hnorm=HNORM):
     return eigensolve_mixed(H, num_vb=4, num_cb=4, tol=1e-8, tol32=TOL32, hnorm=hnorm,
                             max_iter=max_iter, start_iter=200, skip=CHECK_SKIP,
                             start_vector=start, stop_rule="window",
-                            refine_depth=12, refine_rounds=4, **windows)
+                            refine_depth=12, refine_rounds=4,
+                            return_vectors="device", **windows)
 
 
 def main():
@@ -208,25 +246,38 @@ def main():
         sys.exit(2)
 
     t0 = time.time()
-    M, vinfo = build_dot(args.workload)
+    setup = {}
+    M = WORKLOADS[args.workload]["build"]()
+    setup["structure_s"] = time.time() - t0
+    t1 = time.time()
+    vinfo = vff.relax(M)
+    setup["vff_s"] = time.time() - t1
+    t1 = time.time()
     H = Hamiltonian(M, layout=native.LAYOUT_SK, precision=native.PREC_FP64)
     H.set_precision(native.PREC_FP32)  # FP32 copies made outside the timing
     H.set_precision(native.PREC_FP64)
+    setup["assembly_s"] = time.time() - t1
     net = inputs = None
     if args.form != "plain" and os.path.exists(args.weights):
         net, _ = warmstart.load(H, args.weights)
         # the network's per-atom inputs are a property of the structure,
         # built with it (like H) before the timed region
+        t1 = time.time()
         inputs = warmstart.structure_inputs(H)
+        setup["cnn_features_s"] = time.time() - t1
     # warm the kernels (module load, first launches) outside the timing
     eigensolve_lanczos(H, num_vb=0, num_cb=1, erange_cb=(100, 101), max_iter=20,
                        start_iter=1000, want_vectors=False)
     log("setup %.1f s: %d atoms, N=%d, vff %d iterations" % (
         time.time() - t0, M.n_atoms, M.n, vinfo.get("iterations", 0)))
 
+    # the plain solve's eigenvectors stay in device memory (downloaded for
+    # the oracle check after the timed region)
+    pvec = native.DeviceArray(H, (8, M.n)) if args.form != "mixed" else None
+
     def run_plain():
         t = time.time()
-        r = plain_solve(H, args.max_iter, windows)
+        r = plain_solve(H, args.max_iter, windows, vectors_out=pvec)
         return time.time() - t, r
 
     def run_mixed():
@@ -273,6 +324,7 @@ def main():
         "positions": "VFF-relaxed (device Keating, %d CG iterations)"
                      % vinfo.get("iterations", 0),
         "windows_eV": windows,
+        "setup_outside_timing": setup,
         "form": "mixed+learned" if (mixed is not None and net is not None)
                 else ("mixed (random start)" if mixed is not None else "plain"),
         "eigenvalues_eV": np.round(headline["eigenvalues"], 9).tolist(),
@@ -347,7 +399,12 @@ def main():
     if args.workload != "dot786k":
         cfg["lanczos_vector_memory"] = lanczos_memory(M.n)
     if not args.no_cpu_baseline and plain is not None and args.workload == "dot786k":
-        line["cpu_baseline"] = cpu_baseline(M, plain["matvecs"])
+        checks = {"plain": (plain["eigenvalues"],
+                            pvec.download()[:len(plain["eigenvalues"])])}
+        if mixed is not None:
+            dv = mixed["device_vectors"].download()
+            checks["mixed"] = (mixed["eigenvalues"], dv[:len(mixed["order"])][mixed["order"]])
+        line["cpu_baseline"], cfg["oracle_check"] = cpu_leg(M, plain["matvecs"], checks)
     print(json.dumps(line), flush=True)
 
 

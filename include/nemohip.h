@@ -146,6 +146,17 @@ nh_status nh_memcpy_h2d(nh_context *ctx, void *dst, const void *src,
 nh_status nh_memcpy_d2h(nh_context *ctx, void *dst, const void *src,
                         int64_t bytes);
 
+/* ---- the reference's random start vectors (host, no device work) -------
+ * NH_RANDOM_RAND: out[j] = (rand() / RAND_MAX, 0), j < n, from glibc's
+ * rand() after srand(seed[0]) (lanczos_initialize Random,
+ * eigsys3d_par.c:125-131; n = p N gives the block start of
+ * lanczos_blk.c:2220-2223).  NH_RANDOM_ZLARNV: LAPACK zlarnv(1, seed, n,
+ * out), seed[0..3] in 0..4095 with seed[3] odd (qtracemin_driver.c:
+ * 105-116); the seed is not updated. */
+enum { NH_RANDOM_RAND = 0, NH_RANDOM_ZLARNV = 1 };
+nh_status nh_random_start(int kind, const int32_t *seed, int64_t n,
+                          nh_cplx *out);
+
 /* ---- vector kernels (device pointers, length n complex) ---------------
  * Reductions accumulate in FP64 (vect_dot_vect, h_cvectr_mult.c:138-154;
  * Lanczos inner products, lanczos.c:679-754). */
@@ -171,8 +182,10 @@ typedef struct nh_lanczos_opts {
   int32_t conv_check_start;     /* ConvCheckStartIter */
   int32_t conv_check_skip;      /* ConvCheckSkipRate */
   int32_t start;                /* 0 CyclicRandom, 1 Const1, 2 user vector
-                                   (host), 3 user vector (device); built and
-                                   normalised in device memory */
+                                   (host), 3 user vector (device), 4 Random
+                                   ((rand() / RAND_MAX, 0) per entry after
+                                   srand(random_seed), eigsys3d_par.c:
+                                   125-131); normalised in device memory */
   int32_t cyclic_len;           /* CyclicString length */
   const double *cyclic_string;  /* CyclicString values */
   const nh_cplx *start_vector;  /* start == 2 / 3: host / device vector of
@@ -182,6 +195,15 @@ typedef struct nh_lanczos_opts {
                                    1: stop once the num_eig Ritz values
                                    nearest the gap are all converged */
   int32_t want_vectors;         /* compute eigenvectors (second pass) */
+  uint32_t random_seed;         /* start == 4: srand() seed (glibc rand) */
+  /* restart (restoreState, lanczos.c:1905-1960): with restore_len > 0 the
+   * eigenvalue pass is skipped and the tridiagonal alpha[0..len),
+   * beta[0..len) of an earlier run (nh_eig_result.tridiag_*, host arrays)
+   * is analysed instead; the eigenvector pass then starts from the start
+   * vector these options describe, which must be that run's */
+  const double *restore_alpha;
+  const double *restore_beta;
+  int32_t restore_len;
 } nh_lanczos_opts;
 
 typedef struct nh_eig_result {
@@ -223,11 +245,19 @@ typedef struct nh_blanczos_opts {
   double erange_cb[2];
   int32_t conv_check_start;     /* block steps before the first check */
   int32_t conv_check_skip;      /* block steps between checks */
-  uint64_t seed;                /* random start block seed */
+  uint64_t seed;                /* start block without start_block: p vectors
+                                   (rand() / RAND_MAX, 0) in turn after
+                                   srand(seed) (lanczos_blk.c:2214-2229) */
   const nh_cplx *start_block;   /* optional host block [block_size][N] */
   int32_t stop_rule;            /* as nh_lanczos_opts.stop_rule; counts
                                    distinct levels (degenerate copies once) */
   int32_t want_vectors;
+  int32_t deflation_on;         /* DeflationOn: drop a block vector whose
+                                   norm after orthogonalisation falls below
+                                   deflation_tol (deflation_test,
+                                   lanczos_blk.c:225-268); the block then
+                                   shrinks */
+  double deflation_tol;         /* DefTolerance (the deck default is 1e-7) */
 } nh_blanczos_opts;
 
 /* Block Lanczos without reorthogonalisation (eigensolve_blk_lanczos,
@@ -251,7 +281,10 @@ typedef struct nh_tracemin_opts {
   double cgtol;               /* inner CG relative stopping bound */
   int32_t max_outer;          /* maxi */
   int32_t cg_max;             /* cgmaxim */
-  uint64_t seed;
+  uint64_t seed;              /* random block: zlarnv(1, iseed) with iseed =
+                                 {103 + seed, 101 seed, 7, 57} mod 4096, the
+                                 reference's seed of rank `seed`
+                                 (qtracemin_driver.c:105-116) */
   const nh_cplx *start_block; /* optional host block [block_size][N] */
 } nh_tracemin_opts;
 

@@ -26,7 +26,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 
 SOURCES = ["nh_api.hip", "nh_assemble.hip", "nh_hv.hip", "nh_vector.hip",
            "nh_lanczos.hip", "nh_vff.hip", "nh_block.hip", "nh_tracemin.hip", "nh_cnn.hip",
-           "nh_shim.hip", "nh_hv_mfma.hip", "nh_refine.hip", "nh_pctracemin.hip"]
+           "nh_shim.hip", "nh_hv_mfma.hip", "nh_refine.hip", "nh_pctracemin.hip",
+           "nh_random.hip"]
 
 
 def _compile(src):

@@ -387,7 +387,8 @@ nh_status nh_hV(nh_context *ctx, double shift, const nh_cplx *X, nh_cplx *Y,
 
 nh_status nh_ham_export(nh_context *ctx, double *onsite, double *blocks) {
   NH_TRY(check_ctx(ctx));
-  NH_CHECK(ctx->onsite && ctx->blocks64, NH_ERR_STATE, "not built");
+  NH_CHECK(ctx->onsite && ctx->blocks64, NH_ERR_STATE,
+           "nh_ham_export: needs a build in NH_LAYOUT_BLOCKS");
   const int A = ctx->n_atoms;
   if (onsite) {
     std::vector<double> tmp((size_t)A * kOnsiteStride);

@@ -58,11 +58,11 @@ __global__ void k_bond(int n_cations, int S, const int32_t *__restrict__ cations
 
   double v[21];
   scale_integrals(par, strn, g.brat, v);
-  double *Ba = blocks + ((size_t)j * 4 + jslot) * 100;  // anion row: T
-  double *Bc = blocks + ((size_t)i * 4 + slot) * 100;   // cation row: T^T
-  {
-    double T[100];
-    sk_table(g.l, g.m, g.n, v, T);
+  double T[100];
+  sk_table(g.l, g.m, g.n, v, T);
+  if (blocks) {  // stored-block layout only
+    double *Ba = blocks + ((size_t)j * 4 + jslot) * 100;  // anion row: T
+    double *Bc = blocks + ((size_t)i * 4 + slot) * 100;   // cation row: T^T
 #pragma unroll
     for (int a = 0; a < 10; ++a)
 #pragma unroll
@@ -94,7 +94,7 @@ __global__ void k_bond(int n_cations, int S, const int32_t *__restrict__ cations
 #pragma unroll
     for (int c = 0; c < 10; ++c) {
       const int sa = shell_of(a), sc = shell_of(c);
-      const double r1 = Ba[a * 10 + c], r2 = T0[a * 10 + c];
+      const double r1 = T[a * 10 + c], r2 = T0[a * 10 + c];
       const double r3 = (r2 * r2 - r1 * r1) * cs /
                         (par[e_anion(sa)] + par[e_cation(sc)]) *
                         strn[cpair_index(sa, sc)];
@@ -201,6 +201,43 @@ __global__ void k_onsite(int A, int S, const int32_t *__restrict__ species,
   os[17] = 0.0;
 }
 
+// Matrix-free layout tables: per atom its displacement and species, per
+// slot the ideal bond vector seen from the bond's cation (cation rows keep
+// their own vector, anion rows negate theirs), packed as 3 x int8.
+__global__ void k_mf_geo(int A, const int32_t *__restrict__ species,
+                         const int32_t *__restrict__ is_cation,
+                         const int32_t *__restrict__ bond_q,
+                         const double *__restrict__ disp,
+                         double4 *__restrict__ geo, int32_t *__restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A) return;
+  geo[i] = make_double4(disp[i * 3], disp[i * 3 + 1], disp[i * 3 + 2],
+                        (double)species[i]);
+  const int sg = is_cation[i] ? 1 : -1;
+  for (int n = 0; n < 4; ++n) {
+    int packed = 0;
+    for (int k = 0; k < 3; ++k)
+      packed |= ((sg * bond_q[(i * 4 + n) * 3 + k]) & 0xff) << (8 * k);
+    q[i * 4 + n] = packed;
+  }
+}
+
+// Per (cation, anion) species pair: integrals 10..30, strain exponents
+// 0..14, lattice constant (the parameters k_bond reads).
+__global__ void k_mf_pair(int S, const double *__restrict__ parmat,
+                          const double *__restrict__ strnmat,
+                          double *__restrict__ pair) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S * S) return;
+  const double *par = parmat + (size_t)p * 35;
+  const double *strn = strnmat + (size_t)p * 33;
+  double *o = pair + (size_t)p * kMfPair;
+  for (int k = 0; k < 21; ++k) o[k] = par[10 + k];
+  for (int e = 0; e < 15; ++e) o[kMfEta + e] = strn[e];
+  o[kMfA0] = par[P_A0];
+  for (int k = kMfA0 + 1; k < kMfPair; ++k) o[k] = 0.0;
+}
+
 template <typename T>
 __global__ void k_convert(int64_t n, const double *__restrict__ src,
                           T *__restrict__ dst) {
@@ -224,6 +261,12 @@ static void free_ham(nh_context *c) {
   hipFree(c->sk64);
   hipFree(c->sk32);
   hipFree(c->corr);
+  hipFree(c->mf_geo);
+  hipFree(c->mf_q);
+  hipFree(c->mf_pair);
+  c->mf_geo = nullptr;
+  c->mf_q = nullptr;
+  c->mf_pair = nullptr;
   c->onsite = nullptr;
   c->blocks64 = nullptr;
   c->blocks32 = nullptr;
@@ -235,69 +278,71 @@ static void free_ham(nh_context *c) {
 }
 
 nh_status assemble(nh_context *c, int layout, int precision) {
-  NH_CHECK(layout == NH_LAYOUT_BLOCKS || layout == NH_LAYOUT_SK, NH_ERR_ARG,
-           "layout %d not available", layout);
+  NH_CHECK(layout == NH_LAYOUT_BLOCKS || layout == NH_LAYOUT_SK ||
+               layout == NH_LAYOUT_MATRIX_FREE,
+           NH_ERR_ARG, "layout %d not available", layout);
   NH_CHECK(precision == NH_PREC_FP64 || precision == NH_PREC_FP32, NH_ERR_ARG,
            "precision %d not available", precision);
+  NH_HIP(hipStreamSynchronize(c->stream));
   free_ham(c);
   const int A = c->n_atoms;
   const size_t nblk = (size_t)A * 4 * 100;
   NH_TRY(device_malloc(c, (void **)&c->onsite, (size_t)A * kOnsiteStride * sizeof(double)));
-  NH_TRY(device_malloc(c, (void **)&c->blocks64, nblk * sizeof(double)));
   NH_TRY(device_malloc(c, (void **)&c->corr, (size_t)A * 4 * 10 * sizeof(double)));
-  NH_HIP(hipMemsetAsync(c->blocks64, 0, nblk * sizeof(double), c->stream));
   NH_HIP(hipMemsetAsync(c->corr, 0, (size_t)A * 40 * sizeof(double), c->stream));
-  double *sk = nullptr;
+  if (layout == NH_LAYOUT_BLOCKS) {
+    NH_TRY(device_malloc(c, (void **)&c->blocks64, nblk * sizeof(double)));
+    NH_HIP(hipMemsetAsync(c->blocks64, 0, nblk * sizeof(double), c->stream));
+  }
   if (layout == NH_LAYOUT_SK) {
     NH_TRY(device_malloc(c, (void **)&c->sk64, (size_t)c->n_cations * 4 * kSkStride * sizeof(double)));
     NH_HIP(hipMemsetAsync(c->sk64, 0,
                           (size_t)c->n_cations * 4 * kSkStride * sizeof(double),
                           c->stream));
-    sk = c->sk64;
   }
   const int S = c->n_species;
+  if (layout == NH_LAYOUT_MATRIX_FREE) {
+    NH_TRY(device_malloc(c, (void **)&c->mf_geo, (size_t)A * sizeof(double4)));
+    NH_TRY(device_malloc(c, (void **)&c->mf_q, (size_t)A * 4 * sizeof(int32_t)));
+    NH_TRY(device_malloc(c, (void **)&c->mf_pair, (size_t)S * S * kMfPair * sizeof(double)));
+    hipLaunchKernelGGL(k_mf_geo, dim3(grid_for(A, 128)), dim3(128), 0, c->stream,
+                       A, c->species, c->is_cation, c->bond_q, c->disp,
+                       c->mf_geo, c->mf_q);
+    hipLaunchKernelGGL(k_mf_pair, dim3(1), dim3(64 * ((S * S + 63) / 64)), 0,
+                       c->stream, S, c->parmat, c->strnmat, c->mf_pair);
+    NH_HIP(hipGetLastError());
+  }
+  // bonds: the stored blocks or SK records (if kept) and the Boykin terms
+  // of the onsite diagonal, which every layout stores
   hipLaunchKernelGGL(k_bond, dim3(grid_for((int64_t)c->n_cations * 4, 128)),
                      dim3(128), 0, c->stream, c->n_cations, S, c->cations,
                      c->species, c->nbr, c->rev_slot, c->bond_q, c->disp,
                      c->lattice[0], c->lattice[1], c->lattice[2], c->parmat,
-                     c->strnmat, c->blocks64, c->corr, sk);
+                     c->strnmat, c->blocks64, c->corr, c->sk64);
   NH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_onsite, dim3(grid_for(A, 128)), dim3(128), 0, c->stream,
                      A, S, c->species, c->is_cation, c->nbr, c->bond_q,
                      c->parmat, c->strnmat, c->phi, c->passivate, c->db_shift,
                      c->corr, c->onsite);
   NH_HIP(hipGetLastError());
-  if (precision == NH_PREC_FP32) {
-    NH_TRY(device_malloc(c, (void **)&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
-    hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
-                       (int64_t)A * kOnsiteStride, c->onsite, c->onsite32);
-    if (layout == NH_LAYOUT_BLOCKS) {
-      NH_TRY(device_malloc(c, (void **)&c->blocks32, nblk * sizeof(float)));
-      hipLaunchKernelGGL(k_convert<float>, dim3(4096), dim3(256), 0, c->stream,
-                         (int64_t)nblk, c->blocks64, c->blocks32);
-    } else {
-      const int64_t ns = (int64_t)c->n_cations * 4 * kSkStride;
-      NH_TRY(device_malloc(c, (void **)&c->sk32, ns * sizeof(float)));
-      hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
-                         ns, c->sk64, c->sk32);
-    }
-    NH_HIP(hipGetLastError());
-  }
   NH_HIP(hipStreamSynchronize(c->stream));
+  hipFree(c->corr);  // only the assembly needs the per-bond terms
+  c->corr = nullptr;
   c->layout = layout;
-  c->precision = precision;
-  return NH_OK;
+  c->precision = NH_PREC_FP64;
+  return select_precision(c, precision);
 }
 
 nh_status select_precision(nh_context *c, int precision) {
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
-           NH_ERR_STATE, "build the Hamiltonian first");
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || sk_like(c), NH_ERR_STATE,
+           "build the Hamiltonian first");
   NH_CHECK(precision == NH_PREC_FP64 || precision == NH_PREC_FP32, NH_ERR_ARG,
            "precision %d not available", precision);
   const int A = c->n_atoms;
   if (precision == NH_PREC_FP32) {
-    // the FP64 values are always kept: derive the FP32 copies once
-    if (!c->onsite32) {
+    // the FP64 values are always kept: derive the FP32 copies once (the
+    // matrix-free layout needs none)
+    if (c->layout == NH_LAYOUT_BLOCKS && !c->onsite32) {
       NH_TRY(device_malloc(c, (void **)&c->onsite32, (size_t)A * kOnsiteStride * sizeof(float)));
       hipLaunchKernelGGL(k_convert<float>, dim3(1024), dim3(256), 0, c->stream,
                          (int64_t)A * kOnsiteStride, c->onsite, c->onsite32);

@@ -14,6 +14,16 @@
 // calls zhbevx on the band, solve_block_tridiagonal_eigenvalues,
 // lanczos_blk.c:891-991).  Eigenvectors come from a second pass over the
 // stored A_j, B_j.  All vector arithmetic is FP64.
+//
+// Deflation (DeflationOn / DefTolerance, deflation_test, lanczos_blk.c:
+// 225-268): when a column of W (or of the start block) keeps a norm below
+// the tolerance after orthogonalisation against the columns before it
+// (the diagonal of its Cholesky factor), it is dropped and the block
+// shrinks for the rest of the run; B_j then holds the dropped column's
+// coefficients on the new block, restricted to the rows of the columns
+// before it (the rest of that column is below the tolerance).  The
+// reference drops at most one column per step; here every such column of
+// the step goes, one at a time (a start block may hold a level twice).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -47,22 +57,29 @@ class BlockLanczos {
     mem_.push_back(*q);
     return NH_OK;
   }
-  nh_status gram(const double2 *X, const double2 *Y, std::vector<cd> &G);
+  nh_status gram(const double2 *X, int qa, const double2 *Y, int qb,
+                 std::vector<cd> &G);
   nh_status gemm(const double2 *X, int qx, double2 *Y, int qy,
                  const std::vector<cd> &M, double beta);
   struct Factors {
     std::vector<cd> R1, R2;
+    std::vector<int> drop;  // deflation: positions dropped, in order
   };
-  nh_status orthonormalise(double2 *&W, double2 *&Qout, std::vector<cd> &R,
-                           Factors *f = nullptr);
-  nh_status replay_qr(double2 *&W, double2 *&Qout, const Factors &f);
+  nh_status drop_column(double2 *W, int q, int i, int slot);
+  nh_status orthonormalise(double2 *&W, double2 *&Qout, int q, std::vector<cd> &R,
+                           Factors *f, int &qn);
+  nh_status replay_qr(double2 *&W, double2 *&Qout, int q, const Factors &f);
   nh_status step(int j, bool first_pass);
   void analyse(const nh_blanczos_opts *o, double hnorm);
   bool window_done(int w, int req, int rule) const;
 
   nh_context *c_;
   std::vector<void *> mem_;
-  int p_ = 0;
+  int p_ = 0;                // block width of the start block
+  std::vector<int> sz_;      // size of block j (shrinks by deflation)
+  bool deflate_ = false;
+  double dtol_ = 0.0;
+  double2 *Wd_ = nullptr;    // the columns a deflation drops
   int64_t n_ = 0;
   double2 *Qprev_ = nullptr, *Q_ = nullptr, *W_ = nullptr;
   double *dM_ = nullptr, *part_ = nullptr, *sum_ = nullptr;
@@ -76,20 +93,19 @@ class BlockLanczos {
   std::vector<char> lastConv_[2];
 };
 
-nh_status BlockLanczos::gram(const double2 *X, const double2 *Y,
-                             std::vector<cd> &G) {
-  // 8x8 tiles of G = X^H Y (a larger register tile would spill)
-  const int p = p_;
+nh_status BlockLanczos::gram(const double2 *X, int qa_all, const double2 *Y,
+                             int qb_all, std::vector<cd> &G) {
+  // 8x8 tiles of G = X^H Y (qa_all x qb_all; a larger register tile would
+  // spill)
   constexpr int P = 8;
-  G.assign(p * p, 0.0);
+  G.assign(qa_all * qb_all, 0.0);
   std::vector<double> h(2 * P * P);
-  for (int a0 = 0; a0 < p; a0 += P)
-    for (int b0 = 0; b0 < p; b0 += P) {
-      const int qa = std::min(P, p - a0), qb = std::min(P, p - b0);
-      const int q = std::max(qa, qb);
+  for (int a0 = 0; a0 < qa_all; a0 += P)
+    for (int b0 = 0; b0 < qb_all; b0 += P) {
+      const int qa = std::min(P, qa_all - a0), qb = std::min(P, qb_all - b0);
       hipLaunchKernelGGL((k_gram<P>), dim3(nparts_), dim3(256), 0, c_->stream,
-                         n_, q, X + (size_t)a0 * n_, n_, Y + (size_t)b0 * n_,
-                         n_, part_);
+                         n_, qa, qb, X + (size_t)a0 * n_, n_,
+                         Y + (size_t)b0 * n_, n_, part_);
       hipLaunchKernelGGL(k_sum_cols, dim3(sum_cols_grid(2 * P * P)), dim3(256), 0,
                          c_->stream, nparts_, 2 * P * P, part_, sum_);
       NH_HIP(hipGetLastError());
@@ -98,7 +114,8 @@ nh_status BlockLanczos::gram(const double2 *X, const double2 *Y,
       NH_HIP(hipStreamSynchronize(c_->stream));
       for (int a = 0; a < qa; ++a)
         for (int b = 0; b < qb; ++b)
-          G[(a0 + a) * p + b0 + b] = cd(h[2 * (a * P + b)], h[2 * (a * P + b) + 1]);
+          G[(a0 + a) * qb_all + b0 + b] =
+              cd(h[2 * (a * P + b)], h[2 * (a * P + b) + 1]);
     }
   return NH_OK;
 }
@@ -128,64 +145,118 @@ nh_status BlockLanczos::gemm(const double2 *X, int qx, double2 *Y, int qy,
   return NH_OK;
 }
 
+// Removes column i of the q-column block W (columns after it move down by
+// one); the removed column goes to slot `slot` of Wd_.
+nh_status BlockLanczos::drop_column(double2 *W, int q, int i, int slot) {
+  NH_HIP(hipMemcpyAsync(Wd_ + (size_t)slot * n_, W + (size_t)i * n_,
+                        n_ * sizeof(double2), hipMemcpyDeviceToDevice, c_->stream));
+  for (int k = i + 1; k < q; ++k)
+    NH_HIP(hipMemcpyAsync(W + (size_t)(k - 1) * n_, W + (size_t)k * n_,
+                          n_ * sizeof(double2), hipMemcpyDeviceToDevice,
+                          c_->stream));
+  return NH_OK;
+}
+
 // Qout R = W by Cholesky QR twice: Q1 = W R1^-1, Q = Q1 R2^-1, R = R2 R1
 // (upper triangular).  The factors are kept so that the eigenvector pass
 // repeats exactly these two products (a single product with R^-1 differs in
 // rounding, and the difference grows along the recurrence).  The result ends
 // in Qout, W holds Q1 afterwards (the pointers are swapped, no product is
-// done in place).
-nh_status BlockLanczos::orthonormalise(double2 *&W, double2 *&Qout,
-                                       std::vector<cd> &R, Factors *f) {
-  const int p = p_;
+// done in place).  With deflation on, columns whose Cholesky diagonal (the
+// norm after orthogonalisation against the columns before them) is below
+// the tolerance are dropped first, one at a time: qn < q columns come out,
+// and R (qn x q) carries each dropped column's coefficients on the kept
+// columns before it.
+nh_status BlockLanczos::orthonormalise(double2 *&W, double2 *&Qout, int q,
+                                       std::vector<cd> &R, Factors *f, int &qn) {
+  std::vector<int> drop, dropped_orig, kept(q);
+  for (int k = 0; k < q; ++k) kept[k] = k;
+  if (deflate_) {
+    while ((int)kept.size() > 1) {
+      const int qc = (int)kept.size();
+      std::vector<cd> G;
+      NH_TRY(gram(W, qc, W, qc, G));
+      const int d = cholesky_first_small(qc, G, dtol_);
+      if (d < 0) break;
+      NH_TRY(drop_column(W, qc, d, (int)drop.size()));
+      drop.push_back(d);
+      dropped_orig.push_back(kept[d]);
+      kept.erase(kept.begin() + d);
+    }
+  }
+  qn = (int)kept.size();
   std::vector<cd> G[2];
   for (int pass = 0; pass < 2; ++pass) {
     double2 *src = pass == 0 ? W : Qout;
     double2 *dst = pass == 0 ? Qout : W;
-    NH_TRY(gram(src, src, G[pass]));
+    NH_TRY(gram(src, qn, src, qn, G[pass]));
     // tiny diagonal lift keeps Cholesky alive for a nearly dependent block
     double tr = 0.0;
-    for (int k = 0; k < p; ++k) tr += G[pass][k * p + k].real();
-    for (int k = 0; k < p; ++k) G[pass][k * p + k] += 1e-28 * tr;
-    NH_CHECK(cholesky_upper(p, G[pass]), NH_ERR_NOCONV,
-             "block Lanczos: block became rank deficient");
-    NH_TRY(gemm(src, p, dst, p, inv_upper(p, G[pass]), 0.0));
+    for (int k = 0; k < qn; ++k) tr += G[pass][k * qn + k].real();
+    for (int k = 0; k < qn; ++k) G[pass][k * qn + k] += 1e-28 * tr;
+    NH_CHECK(cholesky_upper(qn, G[pass]), NH_ERR_NOCONV,
+             "block Lanczos: block became rank deficient (deflation %s)",
+             deflate_ ? "on" : "off");
+    NH_TRY(gemm(src, qn, dst, qn, inv_upper(qn, G[pass]), 0.0));
   }
   std::swap(W, Qout);  // result (written to W by the second pass) -> Qout
-  R = matmul(p, G[1], G[0]);
+  const std::vector<cd> Rk = matmul(qn, G[1], G[0]);
+  if (drop.empty()) {
+    R = Rk;
+  } else {
+    // coefficients of the dropped columns on the new block, kept on the
+    // rows of the kept columns before each
+    const int nd = (int)drop.size();
+    std::vector<cd> cf;
+    NH_TRY(gram(Qout, qn, Wd_, nd, cf));
+    R.assign((size_t)qn * q, 0.0);
+    for (int k = 0; k < qn; ++k)
+      for (int r = 0; r < qn; ++r) R[r * q + kept[k]] = Rk[r * qn + k];
+    for (int t = 0; t < nd; ++t) {
+      const int c = dropped_orig[t];
+      for (int r = 0; r < qn && kept[r] < c; ++r) R[r * q + c] = cf[r * nd + t];
+    }
+  }
   if (f) {
     f->R1 = G[0];
     f->R2 = G[1];
+    f->drop = drop;
   }
   return NH_OK;
 }
 
-// Replays orthonormalise with stored factors: Qout = (W R1^-1) R2^-1.
-nh_status BlockLanczos::replay_qr(double2 *&W, double2 *&Qout,
+// Replays orthonormalise with stored factors: the same columns dropped,
+// then Qout = (W R1^-1) R2^-1.
+nh_status BlockLanczos::replay_qr(double2 *&W, double2 *&Qout, int q,
                                   const Factors &f) {
-  NH_TRY(gemm(W, p_, Qout, p_, inv_upper(p_, f.R1), 0.0));
-  NH_TRY(gemm(Qout, p_, W, p_, inv_upper(p_, f.R2), 0.0));
+  int qc = q;
+  for (size_t t = 0; t < f.drop.size(); ++t)
+    NH_TRY(drop_column(W, qc--, f.drop[t], (int)t));
+  NH_TRY(gemm(W, qc, Qout, qc, inv_upper(qc, f.R1), 0.0));
+  NH_TRY(gemm(Qout, qc, W, qc, inv_upper(qc, f.R2), 0.0));
   std::swap(W, Qout);
   return NH_OK;
 }
 
 nh_status BlockLanczos::step(int j, bool first_pass) {
-  const int p = p_;
+  const int p = sz_[j];
   // W = H Q_j (a sample of the first-pass products is timed)
   const bool timed = first_pass && j % 4 == 1;
   if (timed) NH_TRY(timer_.begin(c_->stream));
   NH_TRY(launch_hv(c_, 0.0, reinterpret_cast<const nh_cplx *>(Q_),
                    reinterpret_cast<nh_cplx *>(W_), p, n_));
   if (timed) NH_TRY(timer_.end(c_->stream));
-  if (j > 0) {  // W -= Q_{j-1} B_{j-1}^H
+  if (j > 0) {  // W -= Q_{j-1} B_{j-1}^H  (B_{j-1}: p x pm)
+    const int pm = sz_[j - 1];
     const auto &B = T_.B[j - 1];
-    std::vector<cd> M(p * p);
-    for (int k = 0; k < p; ++k)
-      for (int c = 0; c < p; ++c) M[k * p + c] = -std::conj(B[c * p + k]);
-    NH_TRY(gemm(Qprev_, p, W_, p, M, 1.0));
+    std::vector<cd> M(pm * p);
+    for (int k = 0; k < pm; ++k)
+      for (int c = 0; c < p; ++c) M[k * p + c] = -std::conj(B[c * pm + k]);
+    NH_TRY(gemm(Qprev_, pm, W_, p, M, 1.0));
   }
   std::vector<cd> A;
   if (first_pass) {
-    NH_TRY(gram(Q_, W_, A));
+    NH_TRY(gram(Q_, p, W_, p, A));
     for (int a = 0; a < p; ++a)  // Hermitian part
       for (int b = a; b < p; ++b) {
         const cd h = 0.5 * (A[a * p + b] + std::conj(A[b * p + a]));
@@ -199,15 +270,17 @@ nh_status BlockLanczos::step(int j, bool first_pass) {
   std::vector<cd> M(p * p);
   for (int k = 0; k < p * p; ++k) M[k] = -A[k];
   NH_TRY(gemm(Q_, p, W_, p, M, 1.0));
-  std::vector<cd> R;
   // Qprev <- new block (the old Q_{j-1} is no longer needed)
   if (first_pass) {
     Factors f;
-    NH_TRY(orthonormalise(W_, Qprev_, R, &f));
+    std::vector<cd> R;
+    int pn = p;
+    NH_TRY(orthonormalise(W_, Qprev_, p, R, &f, pn));
     T_.B.push_back(R);
     qr_.push_back(f);
+    sz_.push_back(pn);
   } else {
-    NH_TRY(replay_qr(W_, Qprev_, qr_[j]));
+    NH_TRY(replay_qr(W_, Qprev_, p, qr_[j]));
   }
   std::swap(Qprev_, Q_);  // Q_ = Q_{j+1}, Qprev_ = Q_j
   return NH_OK;
@@ -216,6 +289,8 @@ nh_status BlockLanczos::step(int j, bool first_pass) {
 void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
   T_.m = (int)T_.A.size();
   T_.p = p_;
+  T_.sz.assign(sz_.begin(), sz_.begin() + T_.m);
+  T_.layout();
   const double tol = o->conv_tolerance * hnorm;
   // copies of one level closer than this count once: FP64 products keep
   // Kramers partners degenerate to ~1e-12, FP32 products split them by
@@ -224,8 +299,8 @@ void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
   std::vector<cd> work, x;
   const int req[2] = {o->num_eig_vb, o->num_eig_cb};
   const double *win[2] = {o->erange_vb, o->erange_cb};
-  const auto &Bm = T_.B.back();
-  const int p = p_, n = T_.n();
+  const auto &Bm = T_.B.back();  // sz_[m] x sz_[m-1]
+  const int pl = sz_[T_.m - 1], pn = sz_[T_.m], n = T_.n();
   for (int w = 0; w < 2; ++w) {
     lastRitz_[w].clear();
     lastConv_[w].clear();
@@ -254,9 +329,9 @@ void BlockLanczos::analyse(const nh_blanczos_opts *o, double hnorm) {
       band_eigvec(T_, ev, x);
       // residual estimate ||B_m s_m|| (s_m: last block of the Ritz vector)
       double r2 = 0.0;
-      for (int rr = 0; rr < p; ++rr) {
+      for (int rr = 0; rr < pn; ++rr) {
         cd sm = 0.0;
-        for (int cc = 0; cc < p; ++cc) sm += Bm[rr * p + cc] * x[n - p + cc];
+        for (int cc = 0; cc < pl; ++cc) sm += Bm[rr * pl + cc] * x[n - pl + cc];
         r2 += std::norm(sm);
       }
       const bool ok = std::sqrt(r2) <= tol;
@@ -294,7 +369,7 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
   const auto t0 = std::chrono::steady_clock::now();
   p_ = o->block_size;
   NH_CHECK(p_ >= 1 && p_ <= kMaxP, NH_ERR_ARG, "block size must be 1..%d", kMaxP);
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || sk_like(c),
            NH_ERR_STATE, "block Lanczos: build the Hamiltonian first");
   n_ = c->n;
   const size_t blk = (size_t)p_ * n_ * sizeof(double2);
@@ -306,24 +381,25 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
   NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * kMaxP * kMaxP * sizeof(double)));
   NH_TRY(alloc((void **)&sum_, 2 * kMaxP * kMaxP * sizeof(double)));
 
-  // start block: caller's block, or uniform(0,1) real and imaginary parts
-  // from a fixed seed (the reference fills it with zlarnv / rand(),
-  // lanczos_blk.c:2214-2229)
+  // start block: the caller's block, or the reference's random block: p
+  // vectors (rand() / RAND_MAX, 0) in turn after srand(seed)
+  // (lanczos_blk.c:2214-2229)
   std::vector<nh_cplx> S((size_t)p_ * n_);
-  if (o->start_block) {
+  if (o->start_block)
     std::copy(o->start_block, o->start_block + (size_t)p_ * n_, S.begin());
-  } else {
-    uint64_t s = 0x853C49E6748FEA9Bull ^ (uint64_t)o->seed;
-    for (auto &v : S) {
-      s = s * 6364136223846793005ull + 1442695040888963407ull;
-      v.re = (double)(s >> 11) / 9007199254740992.0;
-      s = s * 6364136223846793005ull + 1442695040888963407ull;
-      v.im = (double)(s >> 11) / 9007199254740992.0;
-    }
-  }
+  else
+    rand_start((uint32_t)o->seed, (int64_t)p_ * n_, S.data());
   NH_HIP(hipMemcpyAsync(W_, S.data(), blk, hipMemcpyHostToDevice, c->stream));
+  deflate_ = o->deflation_on != 0;
+  dtol_ = o->deflation_tol;
+  NH_CHECK(!deflate_ || dtol_ >= 0.0, NH_ERR_ARG, "deflation_tol must be >= 0");
+  if (deflate_) NH_TRY(alloc((void **)&Wd_, (size_t)p_ * n_ * sizeof(double2)));
+  // the start block (with deflation on, columns that depend on the ones
+  // before them are dropped: a learned block may hold a level twice)
   std::vector<cd> R0;
-  NH_TRY(orthonormalise(W_, Q_, R0));
+  int qs = p_;
+  NH_TRY(orthonormalise(W_, Q_, p_, R0, nullptr, qs));
+  sz_.assign(1, qs);
   // keep Q_1 for the eigenvector pass
   double2 *Q1 = nullptr;
   NH_TRY(alloc((void **)&Q1, blk));
@@ -335,15 +411,17 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
   Progress prog;
   while (!done && j < o->max_iter) {
     NH_TRY(step(j, true));
+    matvecs += sz_[j];
     ++j;
     prog.tick("block lanczos", j, (int)conv_.size());
     ++skip;
-    matvecs += p_;
     if (j > o->conv_check_start && skip >= o->conv_check_skip) {
       skip = 0;
       // ||H|| estimate from the extreme Ritz values (Gershgorin on T)
       T_.m = j;
       T_.p = p_;
+      T_.sz.assign(sz_.begin(), sz_.begin() + j);
+      T_.layout();
       double g = 0.0;
       for (int r = 0; r < T_.n(); ++r) {
         double s = 0.0;
@@ -395,6 +473,8 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
     // eigenvectors of T (all j blocks), then a second pass accumulating
     // V_e += Q_k s_e[k]
     T_.m = j;
+    T_.sz.assign(sz_.begin(), sz_.begin() + j);
+    T_.layout();
     std::vector<std::vector<cd>> s(ne);
     for (int e = 0; e < ne; ++e) {
       std::vector<const std::vector<cd> *> cluster;
@@ -409,15 +489,16 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
     NH_HIP(hipMemcpyAsync(Q_, Q1, blk, hipMemcpyDeviceToDevice, c->stream));
     const int steps = j;
     for (int k = 0; k < steps; ++k) {
-      // V[:, e] += Q_k s_e[k*p .. k*p+p)
+      // V[:, e] += Q_k s_e[off_k .. off_k + sz_k)
+      const int pk = sz_[k], ok = T_.offset(k);
       for (int e0 = 0; e0 < ne; e0 += kMaxP) {
         const int qe = std::min(kMaxP, ne - e0);
-        std::vector<cd> M(p_ * qe);
-        for (int a = 0; a < p_; ++a)
-          for (int e = 0; e < qe; ++e) M[a * qe + e] = s[e0 + e][k * p_ + a];
+        std::vector<cd> M(pk * qe);
+        for (int a = 0; a < pk; ++a)
+          for (int e = 0; e < qe; ++e) M[a * qe + e] = s[e0 + e][ok + a];
         // gemm writes qe columns with leading dimension n_
-        std::vector<double> h(2 * p_ * qe);
-        for (int t = 0; t < p_ * qe; ++t) {
+        std::vector<double> h(2 * pk * qe);
+        for (int t = 0; t < pk * qe; ++t) {
           h[2 * t] = M[t].real();
           h[2 * t + 1] = M[t].imag();
         }
@@ -426,13 +507,13 @@ nh_status BlockLanczos::run(const nh_blanczos_opts *o, nh_eig_result *res) {
         int64_t g = (n_ + 255) / 256;
         if (g > 16384) g = 16384;
         hipLaunchKernelGGL((k_block_gemm<kMaxP>), dim3((int)g), dim3(256), 0,
-                           c->stream, n_, p_, qe, Q_, n_, V + (size_t)e0 * n_,
+                           c->stream, n_, pk, qe, Q_, n_, V + (size_t)e0 * n_,
                            n_, dM_, 1.0);
         NH_HIP(hipGetLastError());
         NH_HIP(hipStreamSynchronize(c->stream));
       }
       if (k + 1 < steps) NH_TRY(step(k, false));
-      matvecs += p_;
+      matvecs += pk;
     }
     // normalise, Rayleigh quotient, FP64 residual
     double2 *Y = W_;

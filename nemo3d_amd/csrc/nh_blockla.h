@@ -20,10 +20,11 @@ constexpr int kMaxP = 16;
 
 // ---------------- device block kernels ------------------------------------
 
-// part[blk][a*q + b] (re, im) = sum_i conj(X_a[i]) Y_b[i] over this block
+// part[blk][a*P + b] (re, im) = sum_i conj(X_a[i]) Y_b[i] over this block,
+// a < qa, b < qb (the other entries are zero)
 template <int P>
 __global__ __launch_bounds__(256) void k_gram(
-    int64_t n, int q, const double2 *__restrict__ X, int64_t ldx,
+    int64_t n, int qa, int qb, const double2 *__restrict__ X, int64_t ldx,
     const double2 *__restrict__ Y, int64_t ldy, double *__restrict__ part) {
   double re[P * P], im[P * P];
 #pragma unroll
@@ -32,9 +33,9 @@ __global__ __launch_bounds__(256) void k_gram(
        i += (int64_t)gridDim.x * blockDim.x) {
     double2 x[P], y[P];
 #pragma unroll
-    for (int a = 0; a < P; ++a) x[a] = a < q ? X[a * ldx + i] : make_double2(0, 0);
+    for (int a = 0; a < P; ++a) x[a] = a < qa ? X[a * ldx + i] : make_double2(0, 0);
 #pragma unroll
-    for (int b = 0; b < P; ++b) y[b] = b < q ? Y[b * ldy + i] : make_double2(0, 0);
+    for (int b = 0; b < P; ++b) y[b] = b < qb ? Y[b * ldy + i] : make_double2(0, 0);
 #pragma unroll
     for (int a = 0; a < P; ++a)
 #pragma unroll

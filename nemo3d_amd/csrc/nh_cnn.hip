@@ -653,7 +653,7 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
   // C = V^H Y (16 x 8, two 8x8 halves) and the Gram Y^H Y
   std::vector<double> h(2 * 128), hb(2 * 64);
   for (int half = 0; half < 2; ++half) {
-    hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8,
+    hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, 8,
                        V + (int64_t)half * 8 * N, N, Y, N, part);
     hipLaunchKernelGGL(k_sum_cols, dim3(128), dim3(128), 0, s, 512, 128, part, sum);
     hipMemcpyAsync(h.data() + half * 128, sum, 128 * sizeof(double),
@@ -661,7 +661,7 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
     // the next k_sum_cols overwrites `sum`: wait for the copy
     hipStreamSynchronize(s);
   }
-  hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, Y, N, Y, N, part);
+  hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, 8, Y, N, Y, N, part);
   hipLaunchKernelGGL(k_sum_cols, dim3(128), dim3(128), 0, s, 512, 128, part, sum);
   hipMemcpyAsync(hb.data(), sum, 128 * sizeof(double), hipMemcpyDeviceToHost, s);
   hipError_t e = hipStreamSynchronize(s);

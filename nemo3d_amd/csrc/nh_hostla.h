@@ -32,6 +32,28 @@ inline bool cholesky_upper(int p, std::vector<cd> &G) {
   return true;
 }
 
+// First column j of a q-column block, given its Gram matrix G = W^H W,
+// whose norm after orthogonalisation against columns 0..j-1 (the Cholesky
+// diagonal) is below tol, or below 1e-6 of its own norm (the Gram matrix
+// resolves no smaller fraction: its entries carry rounding of order
+// 1e-16 |w|^2); -1 if none.
+inline int cholesky_first_small(int q, std::vector<cd> G, double tol) {
+  for (int j = 0; j < q; ++j) {
+    double d = G[j * q + j].real();
+    const double g = d;
+    for (int k = 0; k < j; ++k) d -= std::norm(G[k * q + j]);
+    if (!(d > tol * tol) || !(d > 1e-12 * g)) return j;
+    d = std::sqrt(d);
+    G[j * q + j] = d;
+    for (int c = j + 1; c < q; ++c) {
+      cd s = G[j * q + c];
+      for (int k = 0; k < j; ++k) s -= std::conj(G[k * q + j]) * G[k * q + c];
+      G[j * q + c] = s / d;
+    }
+  }
+  return -1;
+}
+
 // inverse of upper triangular R
 inline std::vector<cd> inv_upper(int p, const std::vector<cd> &R) {
   std::vector<cd> X(p * p, 0.0);
@@ -55,14 +77,33 @@ inline std::vector<cd> matmul(int p, const std::vector<cd> &A, const std::vector
 }
 
 // Hermitian band matrix from blocks: element (r, c), c >= r, |c - r| <= p.
+// Hermitian block tridiagonal T of m blocks: A_j (s_j x s_j Hermitian,
+// row-major) on the diagonal, B_j (s_{j+1} x s_j, row-major) below it.
+// Blocks are all p wide unless `sz` holds the sizes (a block Lanczos run
+// whose block shrank by deflation); call layout() after changing m or sz.
+// The nonzeros of every row lie within p of the diagonal (p = the largest
+// block; B_j is upper triangular up to the rows a deflation removed).
 struct BandT {
-  int p = 0, m = 0;  // block width, number of blocks
-  std::vector<std::vector<cd>> A, B;  // A_j (Hermitian), B_j (upper)
-  int n() const { return m * p; }
+  int p = 0, m = 0;  // bandwidth (largest block), number of blocks
+  std::vector<std::vector<cd>> A, B;
+  std::vector<int> sz;  // block sizes (empty: every block p)
+  std::vector<int> off_, blk_;
+  void layout() {
+    off_.assign(m + 1, 0);
+    for (int j = 0; j < m; ++j) off_[j + 1] = off_[j] + size(j);
+    blk_.assign(off_[m], 0);
+    for (int j = 0; j < m; ++j)
+      for (int r = off_[j]; r < off_[j + 1]; ++r) blk_[r] = j;
+  }
+  int size(int j) const { return sz.empty() ? p : sz[j]; }
+  int offset(int j) const { return sz.empty() ? j * p : off_[j]; }
+  int n() const { return sz.empty() ? m * p : off_[m]; }
   cd at(int r, int c) const {  // r <= c
-    const int br = r / p, bc = c / p;
-    if (bc == br) return A[br][(r % p) * p + (c % p)];
-    if (bc == br + 1) return std::conj(B[br][(c % p) * p + (r % p)]);  // B^H
+    const int br = sz.empty() ? r / p : blk_[r];
+    const int bc = sz.empty() ? c / p : blk_[c];
+    const int lr = r - offset(br), lc = c - offset(bc), sr = size(br);
+    if (bc == br) return A[br][lr * sr + lc];
+    if (bc == br + 1) return std::conj(B[br][lc * sr + lr]);  // B^H
     return 0.0;
   }
 };

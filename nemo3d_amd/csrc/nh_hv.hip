@@ -6,16 +6,15 @@
 namespace nh {
 namespace {
 
-// SK-compressed layout (see nh_hv_sk.h); workgroup -> cation or anion block
-// by sk_block_of.
-template <int NSP, typename R, bool QUAD>
+// SK-compressed and matrix-free layouts (see nh_hv_sk.h; SRC = SkSrc or
+// MfSrc); workgroup -> cation or anion block by sk_block_of.
+template <int NSP, typename R, bool QUAD, typename SRC>
 __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, int ppx,
-    const double *__restrict__ onsite, const R *__restrict__ sk,
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
-    double shift, const double2 *__restrict__ x, double2 *__restrict__ y,
-    int64_t ld) {
+    const double *__restrict__ onsite, const SRC src,
+    const int32_t *__restrict__ nbr, double shift,
+    const double2 *__restrict__ x, double2 *__restrict__ y, int64_t ld) {
   constexpr int NB = NSP * 10;
   const SkBlock wb = sk_block_of(blockIdx.x, ppx);
   const bool cat = wb.cat;
@@ -34,11 +33,11 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
     L = &stage;
   }
   if (cat)
-    sk_bonds<NSP, R, true, double2, QUAD>(atom, slot, active, sk, nbr, bond_of,
-                                          xc, own_re, own_im, L);
+    sk_bonds<NSP, R, true, double2, QUAD>(atom, slot, active, src, nbr, xc,
+                                          own_re, own_im, L);
   else
-    sk_bonds<NSP, R, false, double2, QUAD>(atom, slot, active, sk, nbr, bond_of,
-                                           xc, own_re, own_im, L);
+    sk_bonds<NSP, R, false, double2, QUAD>(atom, slot, active, src, nbr, xc,
+                                           own_re, own_im, L);
   if constexpr (QUAD) {
     auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
     const int la = threadIdx.x / 4;
@@ -112,33 +111,40 @@ __global__ __launch_bounds__(kThreads) void k_hv_blocks(
 
 nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
                     int ncol, int64_t ld) {
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
-           NH_ERR_STATE, "H·v: Hamiltonian not built in a supported layout");
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || sk_like(c), NH_ERR_STATE,
+           "H·v: Hamiltonian not built in a supported layout");
   NH_CHECK(ncol >= 1 && ncol <= 65535 && ld >= c->n, NH_ERR_ARG,
            "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
   if (hV_mfma_supported(c, ncol)) return launch_hV_mfma(c, shift, x, y, ld);
-  if (c->layout == NH_LAYOUT_SK) {
+  if (sk_like(c)) {
     const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
     const dim3 g(16 * ppx, ncol);
     const double2 *xd = reinterpret_cast<const double2 *>(x);
     double2 *yd = reinterpret_cast<double2 *>(y);
-#define SKL1(NSP, R, SKP, QUAD)                                              \
-  hipLaunchKernelGGL((k_hv_sk<NSP, R, QUAD>), g, dim3(kSkThreads), 0,        \
-                     c->stream, c->n_cations, c->cations, c->n_anions,       \
-                     c->anions, ppx, c->onsite, SKP, c->nbr, c->bond_of,     \
-                     shift, xd, yd, ld)
-#define SKL(NSP, R, SKP)                                                     \
+#define SKL1(NSP, R, SRC, QUAD)                                               \
+  hipLaunchKernelGGL((k_hv_sk<NSP, R, QUAD, decltype(SRC)>), g,               \
+                     dim3(kSkThreads), 0, c->stream, c->n_cations,            \
+                     c->cations, c->n_anions, c->anions, ppx, c->onsite, SRC, \
+                     c->nbr, shift, xd, yd, ld)
+#define SKL(NSP, R, SRC)                                                     \
   do {                                                                       \
-    if (use_staged(c)) SKL1(NSP, R, SKP, true);                               \
-    else SKL1(NSP, R, SKP, false);                                           \
+    if (use_staged(c)) SKL1(NSP, R, SRC, true);                               \
+    else SKL1(NSP, R, SRC, false);                                           \
   } while (0)
-    if (c->precision == NH_PREC_FP64) {
-      if (c->nb == 20) SKL(2, double, c->sk64);
-      else SKL(1, double, c->sk64);
+#define SKP(R, SRC)                                                          \
+  do {                                                                       \
+    if (c->nb == 20) SKL(2, R, SRC);                                         \
+    else SKL(1, R, SRC);                                                     \
+  } while (0)
+    if (c->layout == NH_LAYOUT_MATRIX_FREE) {
+      if (c->precision == NH_PREC_FP64) SKP(double, mf_src<double>(c));
+      else SKP(float, mf_src<float>(c));
+    } else if (c->precision == NH_PREC_FP64) {
+      SKP(double, (SkSrc<double>{c->sk64, c->bond_of}));
     } else {
-      if (c->nb == 20) SKL(2, float, c->sk32);
-      else SKL(1, float, c->sk32);
+      SKP(float, (SkSrc<float>{c->sk32, c->bond_of}));
     }
+#undef SKP
 #undef SKL
 #undef SKL1
     NH_HIP(hipGetLastError());

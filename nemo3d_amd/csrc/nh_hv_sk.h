@@ -1,9 +1,10 @@
-// H·v in the SK-compressed layout: per bond only the direction cosines and
-// the 21 strain-scaled two-centre integrals are stored (24 values, each
-// bond once, cation-owned); the 10x10 block is rebuilt row by row in
-// registers on every product.  The onsite record (diagonal with Boykin
-// corrections, surface couplings, spin-orbit) is stored per atom as in the
-// block layout.
+// H·v in the SK-compressed and matrix-free layouts: per bond only the
+// direction cosines and the 21 strain-scaled two-centre integrals are
+// needed (24 values), stored once per bond (cation-owned) or recomputed
+// from the atoms' displacements (MfSrc); the 10x10 block is rebuilt row by
+// row in registers on every product.  The onsite record (diagonal with
+// Boykin corrections, surface couplings, spin-orbit) is stored per atom as
+// in the block layout.
 //
 // Mapping: 4 lanes per atom, lane n owns neighbour slot n.  A workgroup
 // holds only cation-role or only anion-role atoms (lists `cations` /
@@ -156,6 +157,95 @@ __device__ __forceinline__ void sk_load_record(const float *rec, float (&v)[24])
     v[4 * q + 2] = t.z;
     v[4 * q + 3] = t.w;
   }
+}
+
+// Where a lane's bond record comes from.
+//
+// SkSrc: the stored SK layout, 24 values per bond (each bond once), found
+// through bond_of.
+template <typename R>
+struct SkSrc {
+  const R *__restrict__ sk;
+  const int32_t *__restrict__ bond_of;
+  // FP64: the record is requested before the neighbour gather, whose
+  // barrier its loads then overlap (measured: plain solve -4%); FP32: after
+  // it (requested early, the 24 values spill at 128 VGPRs: +3% per product)
+  static constexpr bool kEarly = sizeof(R) == 8;
+  template <bool CATION>
+  __device__ __forceinline__ void fetch(int atom, int slot, int /*j*/,
+                                        R (&rv)[24]) const {
+    sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
+  }
+};
+
+// MfSrc: the matrix-free layout (the reference's HamCompute = Recompute,
+// Hmult_spds_col, NEMO_3D/nemo3d/src/base/h_cvectr_mult.c:1134-1886): the
+// record is recomputed on every product from the two atoms' displacements
+// and species and the slot's ideal bond vector, as the assembly computes it
+// (strain_corr_prep, mb_ham_spds_micro.c:2848-2978): direction cosines of
+// the strained bond and the 21 integrals scaled by (d0/d)^eta, with
+// (d0/d)^eta = exp(eta log(d0/d)) (one log and 15 exp per bond).  The bond
+// vector is formed in the cation's frame from either end, so both rows of
+// a bond get bit-identical records and H stays exactly Hermitian.  With
+// R = float the integrals are scaled in FP32 (geometry and the log in FP64).
+template <typename R>
+struct MfSrc {
+  const double4 *__restrict__ geo;  // [A] displacement x, y, z (nm), species
+  const int32_t *__restrict__ q;    // [A][4] packed int8 ideal bond vector,
+                                    // cation frame
+  const double *__restrict__ pair;  // [S][S][kMfPair]
+  int S;
+  double lx, ly, lz;
+  static constexpr bool kEarly = true;
+
+  __device__ __forceinline__ static void scale(const double *__restrict__ P,
+                                               double lb, R *__restrict__ v) {
+    R f[15];
+#pragma unroll
+    for (int e = 0; e < 15; ++e) {
+      if constexpr (sizeof(R) == 8) f[e] = exp(P[kMfEta + e] * lb);
+      else f[e] = __expf((float)(P[kMfEta + e] * lb));
+    }
+#pragma unroll
+    for (int k = 0; k < 21; ++k) v[k] = R(P[k]) * f[eta_index(k)];
+  }
+
+  template <bool CATION>
+  __device__ __forceinline__ void fetch(int atom, int slot, int j,
+                                        R (&rv)[24]) const {
+    const double4 gi = geo[atom], gj = geo[j];
+    const int qp = q[atom * 4 + slot];
+    const double4 gc = CATION ? gi : gj, ga = CATION ? gj : gi;
+    const double qx = (double)(int8_t)(qp & 0xff);
+    const double qy = (double)(int8_t)((qp >> 8) & 0xff);
+    const double qz = (double)(int8_t)((qp >> 16) & 0xff);
+    // anion -> cation (getRelativePosVec, geo_struct.c:2278-2294, with the
+    // cation-row sign of h_cvectr_mult.c:1073-1090)
+    const double nx = -(lx * (0.25 * qx) + ga.x - gc.x);
+    const double ny = -(ly * (0.25 * qy) + ga.y - gc.y);
+    const double nz = -(lz * (0.25 * qz) + ga.z - gc.z);
+    const double den = sqrt(nx * nx + ny * ny + nz * nz);
+    rv[0] = R(nx / den);
+    rv[1] = R(ny / den);
+    rv[2] = R(nz / den);
+    const int p = (int)gc.w * S + (int)ga.w;
+    // the common case, one species pair across the wavefront, reads the
+    // pair constants through the scalar cache
+    const int pu = __builtin_amdgcn_readfirstlane(p);
+    if (__all(p == pu)) {
+      const double *P = pair + pu * kMfPair;
+      scale(P, log((0.25 * kSqrt3Ref * P[kMfA0]) / den), rv + 3);
+    } else {
+      const double *P = pair + p * kMfPair;
+      scale(P, log((0.25 * kSqrt3Ref * P[kMfA0]) / den), rv + 3);
+    }
+  }
+};
+
+template <typename R>
+inline MfSrc<R> mf_src(const nh_context *c) {
+  return MfSrc<R>{c->mf_geo, c->mf_q, c->mf_pair, c->n_species,
+                  c->lattice[0], c->lattice[1], c->lattice[2]};
 }
 
 // The neighbour's NB complex entries, from the FP64 vector (one double2
@@ -330,15 +420,15 @@ constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 
 // The 4 bonds of `atom` (lane = slot), summed over the quad in FP64 (fixed
 // order); lane `slot` keeps rows r = 4 i + slot in own_re[i] / own_im[i].
+// SRC: where the bond records come from (SkSrc, MfSrc).
 // XT: double2 (the FP64 vector) or float2 (its FP32 copy, R = float).
 // QUAD: gather through LDS (sk_quad_gather; every thread of the workgroup
 // must call, there is a barrier) in the caller's LDS area L, or directly
 // from the caches.
-template <int NSP, typename R, bool CATION, typename XT, bool QUAD>
+template <int NSP, typename R, bool CATION, typename XT, bool QUAD, typename SRC>
 __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
-                                         const R *__restrict__ sk,
+                                         const SRC &src,
                                          const int32_t *__restrict__ nbr,
-                                         const int32_t *__restrict__ bond_of,
                                          const XT *__restrict__ xc,
                                          double (&own_re)[(NSP * 10 + 3) / 4],
                                          double (&own_im)[(NSP * 10 + 3) / 4],
@@ -351,22 +441,16 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
   const int j = active ? nbr[atom * 4 + slot] : -1;
   R xr[NSP][10], xi[NSP][10];
   if constexpr (QUAD) {
-    // FP64: the bond record is requested before the gather, whose barrier
-    // its loads then overlap (measured: plain solve -4%); FP32: after it
-    // (requested early, the 24 values spill at 128 VGPRs: +3% per product)
     R rv[24];
-    constexpr bool kEarly = sizeof(R) == 8;
-    if (kEarly && j >= 0)
-      sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
+    if (SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
     sk_quad_gather<NSP, R, XT>(j, slot, xc, *L, xr, xi);
-    if (!kEarly && j >= 0)
-      sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
+    if (!SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
     if (j >= 0)
       BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3,
                                                   xr, xi, br, bi);
   } else if (j >= 0) {
     R rv[24];
-    sk_load_record(sk + (size_t)bond_of[atom * 4 + slot] * kSkStride, rv);
+    src.template fetch<CATION>(atom, slot, j, rv);
     sk_load_x<NSP, R>(xc, j, xr, xi);
     BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3, xr,
                                                 xi, br, bi);

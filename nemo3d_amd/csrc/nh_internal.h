@@ -46,6 +46,11 @@ void set_error(const char *fmt, ...);
 constexpr int kOnsiteStride = 18;
 // Per-bond SK record: l, m, n and the 21 scaled integrals (24 values).
 constexpr int kSkStride = 24;
+// Matrix-free layout, per (cation species, anion species) pair: the 21
+// unscaled two-centre integrals (parameters 10..30), the 15 strain
+// exponents (strain parameters 0..14) and the unstrained lattice constant.
+constexpr int kMfPair = 40;
+constexpr int kMfEta = 21, kMfA0 = 36;
 
 }  // namespace nh
 
@@ -93,6 +98,11 @@ struct nh_context {
   double *sk64 = nullptr;       // [Nc*4][kSkStride] (NH_LAYOUT_SK)
   float *sk32 = nullptr;
   double *corr = nullptr;       // [A][4][10] Boykin corrections (scratch)
+  // NH_LAYOUT_MATRIX_FREE: bond records recomputed on every product from
+  double4 *mf_geo = nullptr;    // [A] displacement (nm) x, y, z; species
+  int32_t *mf_q = nullptr;      // [A][4] ideal bond vector of the slot seen
+                                // from its cation, units a/4, 3 x int8
+  double *mf_pair = nullptr;    // [S][S][kMfPair] (cation, anion) constants
 
   // work buffers
   double *red = nullptr;        // reduction partials
@@ -116,8 +126,13 @@ namespace nh {
 // LDS staging of the neighbours' entries in H·v: measured faster for the SK
 // layout (quad gather), slower for the stored-block layout inside the
 // fused Lanczos step; NH_SET_HV_STAGED overrides.
+// Layouts applied by the 4-lanes-per-atom bond kernels (nh_hv_sk.h): the
+// stored SK records and the matrix-free one.
+inline bool sk_like(const nh_context *c) {
+  return c->layout == NH_LAYOUT_SK || c->layout == NH_LAYOUT_MATRIX_FREE;
+}
 inline bool use_staged(const nh_context *c) {
-  return c->hv_staged < 0 ? c->layout == NH_LAYOUT_SK : c->hv_staged != 0;
+  return c->hv_staged < 0 ? sk_like(c) : c->hv_staged != 0;
 }
 }  // namespace nh
 
@@ -201,6 +216,22 @@ template <typename T>
 inline nh_status device_alloc(nh_context *c, T **p, size_t count) {
   return device_malloc(c, reinterpret_cast<void **>(p), count * sizeof(T));
 }
+// The reference's random starts on the host (nh_random.hip): glibc's
+// rand() after srand(seed), LAPACK's dlaruv / zlarnv(idist = 1), and the
+// Lanczos Random start (rand() / RAND_MAX, 0) built from them.
+class GlibcRand {
+ public:
+  explicit GlibcRand(uint32_t seed);
+  int32_t next() { return (int32_t)(step() >> 1); }
+
+ private:
+  uint32_t step();
+  uint32_t r_[34];
+  int k_ = 0;
+};
+void dlaruv(int32_t iseed[4], int n, double *x);
+void zlarnv_uniform(int32_t iseed[4], int64_t n, nh_cplx *x);
+void rand_start(uint32_t seed, int64_t n, nh_cplx *x);
 // device memory from the context's workspace pool: the smallest free block
 // of at least `bytes` and at most twice that, else a new allocation (after
 // returning the free blocks to the device if it is full); pool_release

@@ -140,13 +140,12 @@ __device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
                          al, be, invp);
 }
 
-template <int NSP, typename R, bool QUAD, bool FUSE>
+template <int NSP, typename R, bool QUAD, bool FUSE, typename SRC>
 __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, int ppx,
-    const double *__restrict__ onsite, const R *__restrict__ sk,
-    const int32_t *__restrict__ nbr, const int32_t *__restrict__ bond_of,
-    const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
+    const double *__restrict__ onsite, const SRC src,
+    const int32_t *__restrict__ nbr, const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
     double2 *__restrict__ Q, double2 *__restrict__ Y,
     const double *__restrict__ beta_prev, double *__restrict__ part,
     const double *__restrict__ alpha, const double2 *__restrict__ Rp,
@@ -179,11 +178,11 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
     L = &stage;
   }
   if (cat)
-    sk_bonds<NSP, R, true, XT, QUAD>(atom, slot, active, sk, nbr, bond_of, xg,
-                                     own_re, own_im, L);
+    sk_bonds<NSP, R, true, XT, QUAD>(atom, slot, active, src, nbr, xg, own_re,
+                                     own_im, L);
   else
-    sk_bonds<NSP, R, false, XT, QUAD>(atom, slot, active, sk, nbr, bond_of, xg,
-                                      own_re, own_im, L);
+    sk_bonds<NSP, R, false, XT, QUAD>(atom, slot, active, src, nbr, xg, own_re,
+                                      own_im, L);
   double local = 0.0;
   if constexpr (QUAD) {
     static_assert(sizeof(SkOwnStage<NSP>) <= sizeof(SkQuadStage<NSP, XT>),
@@ -651,6 +650,14 @@ nh_status LanczosRun::build_start(const nh_lanczos_opts *o) {
     NH_HIP(hipGetLastError());
     // the host CyclicString may be released on return
     NH_HIP(hipStreamSynchronize(c_->stream));
+  } else if (o->start == 4) {
+    // Random: glibc rand() after srand(random_seed), drawn on the host in
+    // the reference's order (one number per entry, real part)
+    std::vector<nh_cplx> h((size_t)n_);
+    rand_start(o->random_seed, n_, h.data());
+    NH_HIP(hipMemcpyAsync(R0_, h.data(), n_ * sizeof(double2),
+                          hipMemcpyHostToDevice, c_->stream));
+    NH_HIP(hipStreamSynchronize(c_->stream));  // h is released on return
   } else {
     NH_CHECK(o->start == 2 || o->start == 3, NH_ERR_ARG, "bad start %d", o->start);
     NH_CHECK(o->start_vector, NH_ERR_ARG, "start vector missing");
@@ -681,7 +688,7 @@ nh_status LanczosRun::upload_start(float2 *f32) {
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
   const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
-  const int gridA = c->layout == NH_LAYOUT_SK
+  const int gridA = sk_like(c)
                         ? 16 * ppx
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   // q_k is stored (ring slot) only in the eigenvector pass, which
@@ -706,7 +713,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
   double *part = red ? part_ : nullptr;
   const bool timed = red && (k % kTimeEvery) == 0;
   // the eigenvector pass knows alpha_k: the SK kernel applies the update
-  const bool fuse = !red && c->layout == NH_LAYOUT_SK;
+  const bool fuse = !red && sk_like(c);
   std::pair<hipEvent_t, hipEvent_t> evp;
   if (timed) {
     NH_HIP(hipEventCreate(&evp.first));
@@ -724,30 +731,33 @@ nh_status LanczosRun::iterate(int k, bool red) {
                          dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
                          BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
   } while (0)
-#define LZ_SK2(NSP, R, SKP, QUAD, FUSE)                                         \
-  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE>), dim3(gridA),             \
-                     dim3(kSkThreads), 0, c->stream, c->n_cations, c->cations,  \
-                     c->n_anions, c->anions, ppx, c->onsite, SKP, c->nbr,       \
-                     c->bond_of, Rk, g32, Q, Yk, bprev, part, alpha_ + k,       \
+#define LZ_SK2(NSP, R, SRC, QUAD, FUSE)                                         \
+  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE, decltype(SRC)>),           \
+                     dim3(gridA), dim3(kSkThreads), 0, c->stream, c->n_cations, \
+                     c->cations, c->n_anions, c->anions, ppx, c->onsite, SRC,   \
+                     c->nbr, Rk, g32, Q, Yk, bprev, part, alpha_ + k,           \
                      k > 0 ? Rkm1 : nullptr, bprev2, y32)
-#define LZ_SK1(NSP, R, SKP, QUAD)                                               \
+#define LZ_SK1(NSP, R, SRC, QUAD)                                               \
   do {                                                                          \
-    if (fuse) LZ_SK2(NSP, R, SKP, QUAD, true);                                  \
-    else LZ_SK2(NSP, R, SKP, QUAD, false);                                      \
+    if (fuse) LZ_SK2(NSP, R, SRC, QUAD, true);                                  \
+    else LZ_SK2(NSP, R, SRC, QUAD, false);                                      \
   } while (0)
-#define LZ_SK(NSP, R, SKP)                                                      \
+#define LZ_SK0(NSP, R, SRC)                                                     \
   do {                                                                          \
-    if (use_staged(c)) LZ_SK1(NSP, R, SKP, true);                                \
-    else LZ_SK1(NSP, R, SKP, false);                                            \
+    if (use_staged(c)) LZ_SK1(NSP, R, SRC, true);                                \
+    else LZ_SK1(NSP, R, SRC, false);                                            \
   } while (0)
-  if (c->layout == NH_LAYOUT_SK) {
-    if (c->precision == NH_PREC_FP64) {
-      if (c->nb == 20) LZ_SK(2, double, c->sk64);
-      else LZ_SK(1, double, c->sk64);
-    } else {
-      if (c->nb == 20) LZ_SK(2, float, c->sk32);
-      else LZ_SK(1, float, c->sk32);
-    }
+#define LZ_SK(R, SRC)                                                           \
+  do {                                                                          \
+    if (c->nb == 20) LZ_SK0(2, R, SRC);                                         \
+    else LZ_SK0(1, R, SRC);                                                     \
+  } while (0)
+  if (c->layout == NH_LAYOUT_MATRIX_FREE) {
+    if (c->precision == NH_PREC_FP64) LZ_SK(double, mf_src<double>(c));
+    else LZ_SK(float, mf_src<float>(c));
+  } else if (c->layout == NH_LAYOUT_SK) {
+    if (c->precision == NH_PREC_FP64) LZ_SK(double, (SkSrc<double>{c->sk64, c->bond_of}));
+    else LZ_SK(float, (SkSrc<float>{c->sk32, c->bond_of}));
   } else if (c->precision == NH_PREC_FP64) {
     if (c->nb == 20) LZ_HV(2, double, c->blocks64);
     else LZ_HV(1, double, c->blocks64);
@@ -757,6 +767,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
   }
 #undef LZ_HV
 #undef LZ_SK
+#undef LZ_SK0
 #undef LZ_SK1
 #undef LZ_SK2
   if (timed) {
@@ -897,8 +908,8 @@ nh_status LanczosRun::analyse(int m, const nh_lanczos_opts *o) {
 nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   nh_context *c = c_;
   const auto t0 = std::chrono::steady_clock::now();
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
-           NH_ERR_STATE, "Lanczos: build the Hamiltonian first");
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || sk_like(c), NH_ERR_STATE,
+           "Lanczos: build the Hamiltonian first");
   NH_CHECK(o->max_iter > 2 && o->conv_tolerance > 0, NH_ERR_ARG,
            "Lanczos: bad max_iter / tolerance");
   n_ = c->n;
@@ -912,7 +923,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * sizeof(double)));
 
   NH_TRY(build_start(o));
-  if (c->layout == NH_LAYOUT_SK && c->precision == NH_PREC_FP32) {
+  if (sk_like(c) && c->precision == NH_PREC_FP32) {
     NH_TRY(alloc((void **)&R32_, n_ * sizeof(float2)));
     NH_TRY(alloc((void **)&Y32_, n_ * sizeof(float2)));
   }
@@ -932,7 +943,23 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   bool done = false;
   int matvecs = 0;
   Progress prog;
-  while (m + 1 < o->max_iter) {
+  if (o->restore_len > 0) {
+    // restart from the tridiagonal of an earlier run (restoreState,
+    // lanczos.c:1905-1960): no eigenvalue pass
+    NH_CHECK(o->restore_alpha && o->restore_beta && o->restore_len <= o->max_iter,
+             NH_ERR_ARG, "Lanczos restart: tridiagonal missing or longer than max_iter");
+    m = o->restore_len;
+    NH_HIP(hipMemcpyAsync(alpha_, o->restore_alpha, (size_t)m * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
+    NH_HIP(hipMemcpyAsync(beta_, o->restore_beta, (size_t)m * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
+    NH_TRY(fetch(m));
+    NH_TRY(analyse(m, o));
+    done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+           window_done(1, o->num_eig_cb, o->stop_rule);
+    pending = m;
+  }
+  while (o->restore_len <= 0 && m + 1 < o->max_iter) {
     NH_TRY(iterate(m, true));
     ++m;
     prog.tick("lanczos", m, (int)conv_.size());
@@ -952,7 +979,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   }
   if (done) {
     m = pending;
-  } else {
+  } else if (o->restore_len <= 0) {
     NH_TRY(fetch(m));
     NH_TRY(analyse(m, o));
     done = window_done(0, o->num_eig_vb, o->stop_rule) &&
@@ -1022,7 +1049,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     // free memory (the run's buffers are released before any later stage
     // allocates), else 32, 16 or 8 (each flush reads and writes the
     // eigenvectors once)
-    const bool rring = !R32_ && c->layout == NH_LAYOUT_SK;
+    const bool rring = !R32_ && sk_like(c);
     {
       size_t fr = 0, tot = 0;
       NH_HIP(hipMemGetInfo(&fr, &tot));

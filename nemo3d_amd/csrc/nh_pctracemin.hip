@@ -123,20 +123,18 @@ nh_status PCTracemin::run(nh_eig_result *res) {
   NH_TRY(ops_.alloc((void **)&part_, (size_t)nparts_ * 2 * 16 * sizeof(double)));
   NH_TRY(ops_.alloc((void **)&sum_, 2 * 16 * sizeof(double)));
 
-  // start block: caller's, or uniform (0,1) real and imaginary parts from a
-  // fixed seed (the reference draws zlarnv(idist=1), qtracemin_driver.c)
+  // start block: the caller's, or the reference's random block (uniform
+  // (0,1) real and imaginary parts from LAPACK zlarnv, as for PQ-TRACEMIN)
   {
     std::vector<nh_cplx> S((size_t)p_ * n_);
     if (o->start_block) {
       std::copy(o->start_block, o->start_block + (size_t)p_ * n_, S.begin());
     } else {
-      uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)o->seed;
-      for (auto &v : S) {
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        v.re = (double)(st >> 11) / 9007199254740992.0;
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        v.im = (double)(st >> 11) / 9007199254740992.0;
-      }
+      // zlarnv(1, iseed, n, Y), iseed = {103 + rank, 101 rank, 7, 57}
+      // (qtracemin_driver.c:105-116) with rank = seed
+      const uint64_t r = o->seed;
+      int32_t iseed[4] = {(int32_t)((103 + r) % 4096), (int32_t)((101 * r) % 4096), 7, 57};
+      zlarnv_uniform(iseed, (int64_t)S.size(), S.data());
     }
     NH_HIP(hipMemcpyAsync(W1_, S.data(), S.size() * sizeof(double2),
                           hipMemcpyHostToDevice, c->stream));
@@ -246,7 +244,7 @@ extern "C" nh_status nh_eigensolve_pctracemin(nh_context *ctx,
                                               nh_eig_result *res) {
   NH_CHECK(ctx && opts && res && res->eigenvalues && res->capacity >= 1,
            NH_ERR_ARG, "PC-TRACEMIN: null argument");
-  NH_CHECK(ctx->layout == NH_LAYOUT_BLOCKS || ctx->layout == NH_LAYOUT_SK,
+  NH_CHECK(ctx->layout == NH_LAYOUT_BLOCKS || sk_like(ctx),
            NH_ERR_STATE, "PC-TRACEMIN: build the Hamiltonian first");
   NH_HIP(hipSetDevice(ctx->device));
   res->n_found = 0;

@@ -257,7 +257,7 @@ extern "C" nh_status nh_refine_eigenpairs(nh_context *ctx,
            k, o->depth, o->max_rounds);
   NH_CHECK(ptr_kind == NH_PTR_HOST || ptr_kind == NH_PTR_DEVICE, NH_ERR_ARG,
            "refine: bad ptr_kind %d", ptr_kind);
-  NH_CHECK(ctx->layout == NH_LAYOUT_BLOCKS || ctx->layout == NH_LAYOUT_SK,
+  NH_CHECK(ctx->layout == NH_LAYOUT_BLOCKS || sk_like(ctx),
            NH_ERR_STATE, "refine: build the Hamiltonian first");
   NH_CHECK(ctx->precision == NH_PREC_FP64, NH_ERR_STATE,
            "refine: the operator must be FP64 (nh_ham_set_precision)");

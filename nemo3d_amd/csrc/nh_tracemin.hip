@@ -61,7 +61,7 @@ class Tracemin {
       for (int b0 = 0; b0 < s_; b0 += P) {
         const int qa = std::min(P, s_ - a0), qb = std::min(P, s_ - b0);
         hipLaunchKernelGGL((k_gram<P>), dim3(kParts), dim3(256), 0, c_->stream,
-                           n_, std::max(qa, qb), X + (size_t)a0 * n_, n_,
+                           n_, qa, qb, X + (size_t)a0 * n_, n_,
                            Y + (size_t)b0 * n_, n_, part_);
         hipLaunchKernelGGL(k_sum_cols, dim3(sum_cols_grid(2 * P * P)), dim3(256), 0,
                            c_->stream, kParts, 2 * P * P, part_, sum_);
@@ -152,7 +152,7 @@ class Tracemin {
 nh_status Tracemin::run(const nh_tracemin_opts *o, nh_eig_result *res) {
   nh_context *c = c_;
   const auto t0 = std::chrono::steady_clock::now();
-  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || c->layout == NH_LAYOUT_SK,
+  NH_CHECK(c->layout == NH_LAYOUT_BLOCKS || sk_like(c),
            NH_ERR_STATE, "TRACEMIN: build the Hamiltonian first");
   s_ = o->block_size > 0 ? o->block_size : std::min(kMaxP, 2 * o->num_eig);
   NH_CHECK(o->num_eig >= 1 && s_ >= o->num_eig && s_ <= kMaxP, NH_ERR_ARG,
@@ -179,20 +179,18 @@ nh_status Tracemin::run(const nh_tracemin_opts *o, nh_eig_result *res) {
   NH_TRY(device_malloc(c_, (void **)&dAB_, 2 * kMaxP * sizeof(double)));
   mem_.push_back(dAB_);
 
-  // start block: caller's, or uniform (0,1) real/imag parts (zlarnv idist=1,
-  // qtracemin_driver.c:112-116) from a fixed seed
+  // start block: the caller's, or the reference's random block (uniform
+  // (0,1) real and imaginary parts from LAPACK zlarnv)
   {
     std::vector<nh_cplx> S((size_t)s_ * n_);
     if (o->start_block) {
       std::copy(o->start_block, o->start_block + (size_t)s_ * n_, S.begin());
     } else {
-      uint64_t st = 0xDA3E39CB94B95BDBull ^ (uint64_t)o->seed;
-      for (auto &v : S) {
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        v.re = (double)(st >> 11) / 9007199254740992.0;
-        st = st * 6364136223846793005ull + 1442695040888963407ull;
-        v.im = (double)(st >> 11) / 9007199254740992.0;
-      }
+      // zlarnv(1, iseed, n, Y), iseed = {103 + rank, 101 rank, 7, 57}
+      // (qtracemin_driver.c:105-116) with rank = seed
+      const uint64_t r = o->seed;
+      int32_t iseed[4] = {(int32_t)((103 + r) % 4096), (int32_t)((101 * r) % 4096), 7, 57};
+      zlarnv_uniform(iseed, (int64_t)S.size(), S.data());
     }
     NH_CHECK(Y && !S.empty(), NH_ERR_STATE,
              "TRACEMIN: start block of %d x %lld not allocated", s_, (long long)n_);

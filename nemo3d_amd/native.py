@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libnemohip.so")
 
 NH_OK = 0
 LAYOUT_BLOCKS, LAYOUT_SK, LAYOUT_MATRIX_FREE = 0, 1, 2
+RANDOM_RAND, RANDOM_ZLARNV = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
 SET_HV_STAGED = 1
@@ -67,6 +68,10 @@ class NHLanczosOpts(ctypes.Structure):
         ("start_vector", ctypes.c_void_p),
         ("stop_rule", ctypes.c_int32),
         ("want_vectors", ctypes.c_int32),
+        ("random_seed", ctypes.c_uint32),
+        ("restore_alpha", ctypes.c_void_p),
+        ("restore_beta", ctypes.c_void_p),
+        ("restore_len", ctypes.c_int32),
     ]
 
 
@@ -85,6 +90,8 @@ class NHBLanczosOpts(ctypes.Structure):
         ("start_block", ctypes.c_void_p),
         ("stop_rule", ctypes.c_int32),
         ("want_vectors", ctypes.c_int32),
+        ("deflation_on", ctypes.c_int32),
+        ("deflation_tol", ctypes.c_double),
     ]
 
 
@@ -159,6 +166,7 @@ SIGNATURES = {
                              ctypes.c_int64, ctypes.c_int]),
     "nh_ham_export": (ctypes.c_int, [P, P, P]),
     "nh_context_mem_info": (ctypes.c_int, [P, P, P]),
+    "nh_random_start": (ctypes.c_int, [ctypes.c_int, P, ctypes.c_int64, P]),
     "nh_device_alloc": (ctypes.c_int, [P, ctypes.c_int64, P]),
     "nh_device_free": (ctypes.c_int, [P, P]),
     "nh_memcpy_h2d": (ctypes.c_int, [P, P, P, ctypes.c_int64]),

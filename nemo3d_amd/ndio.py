@@ -173,3 +173,30 @@ def read_ek(path):
 def read_evec(path):
     d = read(path).data
     return d["wf.r"] + 1j * d["wf.i"]
+
+
+def write_tridiagonal(path, alpha, beta, processors=1):
+    """The Lanczos tridiagonal in the reference's binary checkpoint
+    (saveTridiagonalMatrix, NEMO_3D/eigen/src/lanczos.c:1692-1714): int32
+    iterations, int32 processors, then 2 * iterations doubles alpha_0,
+    beta_0, alpha_1, beta_1, ...; the last slot (beta of the final step,
+    which the reference leaves unset) holds beta_(n-1)."""
+    a = np.asarray(alpha, dtype="<f8")
+    b = np.asarray(beta, dtype="<f8")
+    n = a.size
+    t = np.empty(2 * n, dtype="<f8")
+    t[0::2] = a
+    t[1::2] = b[:n]
+    with open(path, "wb") as f:
+        f.write(np.array([n, processors], dtype="<i4").tobytes())
+        f.write(t.tobytes())
+
+
+def read_tridiagonal(path):
+    """(alpha, beta, processors) of a checkpoint written by
+    write_tridiagonal or the reference (restoreTridiagonalMatrix)."""
+    raw = open(path, "rb").read()
+    n, procs = np.frombuffer(raw[:8], dtype="<i4")
+    t = np.frombuffer(raw[8:8 + 16 * int(n)], dtype="<f8")
+    return t[0::2].copy(), t[1::2].copy(), int(procs)
+

@@ -14,7 +14,8 @@ import numpy as np
 from . import native
 
 DEFAULT_CYCLIC = (1.2, 0.8, -1.4, 0.1, -0.9, -0.2, 0.6)
-START = {"CyclicRandom": 0, "Const1": 1, "Vector": 2, "DeviceVector": 3}
+START = {"CyclicRandom": 0, "Const1": 1, "Vector": 2, "DeviceVector": 3,
+         "Random": 4}
 
 
 def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
@@ -23,11 +24,17 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                        cyclic=DEFAULT_CYCLIC, start_vector=None,
                        want_vectors=True, capacity=64, return_vectors=False,
                        return_tridiag=False, stop_rule="window",
-                       vectors_out=None):
+                       vectors_out=None, random_seed=1, restore=None):
     """stop_rule "window": stop once num_vb / num_cb converged values lie in
     the windows (the reference's rule, lanczos.c:1231-1232); "nearest":
     stop once the num_vb / num_cb Ritz values nearest the gap are all
-    converged."""
+    converged.  start "Random": (rand() / RAND_MAX, 0) per entry after
+    srand(random_seed), glibc's rand() (eigsys3d_par.c:125-131).
+    restore=(alpha, beta): restart from the tridiagonal of an earlier run
+    with the same start (the result's "alpha"/"beta" with
+    return_tridiag=True, or ndio.read_tridiagonal of a saved one): no
+    eigenvalue pass, only the analysis and the eigenvector pass
+    (restoreState, lanczos.c:1905-1960)."""
     lib = native.load()
     o = native.NHLanczosOpts()
     o.max_iter = int(max_iter)
@@ -53,6 +60,13 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
         o.start_vector = native.ptr(sv)
     o.stop_rule = {"window": 0, "nearest": 1}[stop_rule]
     o.want_vectors = int(bool(want_vectors))
+    o.random_seed = int(random_seed)
+    if restore is not None:
+        ra = np.ascontiguousarray(restore[0], dtype=np.float64)
+        rb = np.ascontiguousarray(restore[1], dtype=np.float64)
+        assert ra.shape == rb.shape and ra.size <= max_iter
+        o.restore_alpha, o.restore_beta = native.ptr(ra), native.ptr(rb)
+        o.restore_len = ra.size
     ev = np.zeros(capacity)
     rs = np.full(capacity, -1.0)
     vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
@@ -91,10 +105,15 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
                              max_iter=2000, tol=1e-8, start_iter=20, skip=10,
                              seed=1, start_block=None, want_vectors=True,
                              capacity=64, return_vectors=False,
-                             stop_rule="window"):
+                             stop_rule="window", deflation=False,
+                             deflation_tol=1e-7):
     """Block Lanczos (eigensolve_blk_lanczos, NEMO_3D/eigen/src/
     lanczos_blk.c:2145-...) with BL_size = block_size.  num_vb / num_cb
-    count distinct levels; stop_rule as in eigensolve_lanczos."""
+    count distinct levels; stop_rule as in eigensolve_lanczos.  Without a
+    start block the reference's random block (rand() after srand(seed)).
+    deflation / deflation_tol: DeflationOn / DefTolerance (a block vector
+    whose norm after orthogonalisation is below the tolerance is dropped
+    and the block shrinks, deflation_test, lanczos_blk.c:225-268)."""
     lib = native.load()
     o = native.NHBLanczosOpts()
     o.block_size = int(block_size)
@@ -113,6 +132,8 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
         o.start_block = native.ptr(sb)
     o.stop_rule = {"window": 0, "nearest": 1}[stop_rule]
     o.want_vectors = int(bool(want_vectors))
+    o.deflation_on = int(bool(deflation))
+    o.deflation_tol = float(deflation_tol)
     ev = np.zeros(capacity)
     rs = np.full(capacity, -1.0)
     vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
@@ -136,10 +157,12 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
 
 def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
                         hnorm=40.0, cgtol=1.0, max_outer=40, cg_max=100,
-                        seed=1, start_block=None, capacity=64,
+                        seed=0, start_block=None, capacity=64,
                         return_vectors=False):
     """PQ-TRACEMIN on (H - ref_eig)^2 (qtracemin_driver,
-    NEMO_3D/nemo3d/src/base/qtracemin_driver.c:9-947)."""
+    NEMO_3D/nemo3d/src/base/qtracemin_driver.c:9-947).  Without a start
+    block the reference's zlarnv block of rank `seed` (iseed = {103 +
+    seed, 101 seed, 7, 57})."""
     lib = native.load()
     o = native.NHTraceminOpts()
     o.num_eig = int(num_eig)
@@ -176,7 +199,7 @@ def eigensolve_tracemin(H, num_eig=4, ref_eig=0.7, block_size=0, tol=1e-8,
 
 def eigensolve_pctracemin(H, num_eig=4, want_range=(0.6, 0.7),
                           all_range=(-13.0, 40.0), block_size=0, cheb_degree=20,
-                          tol=1e-8, hnorm=40.0, max_outer=200, seed=1,
+                          tol=1e-8, hnorm=40.0, max_outer=200, seed=0,
                           start_block=None, capacity=16, return_vectors=False):
     """PC-TRACEMIN (ctracemin_driver, NEMO_3D/nemo3d/src/base/
     ctracemin_driver.c): subspace iteration on T_k(1 - gamma (H-a)(H-b)) + 2I
@@ -259,7 +282,9 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     with FP32 H·v (FP32 values and products inside each block, FP64 sums
     over the bonds) while the recurrence, dots and the tridiagonal stay
     FP64; then FP64 residual refinement (refine_eigenpairs) of its Ritz
-    pairs to ||H x - e x|| <= tol * hnorm.  tol32: convergence tolerance of
+    pairs to ||H x - e x|| <= tol * hnorm.  return_vectors: True gives the
+    eigenvectors on the host, "device" the native.DeviceArray that holds
+    them (rows in the order of `order` in the result).  tol32: convergence tolerance of
     the FP32 stage (default tol).  refine_split: refine the valence and the
     conduction pairs as two blocks (default) instead of one.  start_vector:
     optional start, e.g. from
@@ -316,7 +341,10 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                                           "eigenvalues", "residuals")},
            "refine": {k: rf[k] for k in ("rounds", "matvecs", "seconds",
                                          "converged")}}
-    if return_vectors:
+    if return_vectors == "device":
+        out["device_vectors"] = dv
+        out["order"] = order
+    elif return_vectors:
         out["vectors"] = dv.download()[:len(order)][order] if n else np.zeros((0, H.n))
     return out
 

@@ -12,6 +12,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* The reference writes sqrt(3) as this truncated literal
  * (nemo3d/src/ham/mb_ham_spds_micro.c:42).  It enters every d-orbital
@@ -411,12 +414,32 @@ static void onsite_apply(int nb, const double *os, const double *x,
   }
 }
 
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+#else
+  (void)n;
+#endif
+}
+
+int or_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 void or_hv(int nb, int n_atoms, const double *onsite, const double *blocks,
            const int *nbr, const double *phase, double shift, const double *x,
            double *y) {
-  int i, n, s, o, k;
+  int i;
   const int nsp = nb / 10;
+  /* every atom's rows are computed independently (row-owner layout), so
+   * the result does not depend on the thread count */
+#pragma omp parallel for schedule(static)
   for (i = 0; i < n_atoms; ++i) {
+    int n, s, o, k;
     const double *xi = x + (size_t)i * nb * 2;
     double *yi = y + (size_t)i * nb * 2;
     for (k = 0; k < 2 * nb; ++k) yi[k] = 0.0;
@@ -461,18 +484,39 @@ void or_lanczos_tridiag(int nb, int n_atoms, const double *onsite,
   double b = 1.0;
   size_t j;
   int it;
+  /* the sums run over a fixed number of contiguous chunks, added in chunk
+   * order: the result does not depend on the thread count */
+  enum { NCH = 256 };
+  double part[NCH];
+  int ch;
   memcpy(r, r0, 2 * N * sizeof(double));
   for (it = 0; it < iters; ++it) {
     const double inv = 1.0 / b;
     double al = 0.0, bb = 0.0, bprev = it ? b : 0.0;
+#pragma omp parallel for schedule(static)
     for (j = 0; j < 2 * N; ++j) q[j] = r[j] * inv;
     or_hv(nb, n_atoms, onsite, blocks, nbr, NULL, 0.0, q, r);
-    for (j = 0; j < N; ++j) al += r[2 * j] * q[2 * j] + r[2 * j + 1] * q[2 * j + 1];
-    for (j = 0; j < 2 * N; ++j) {
-      r[j] -= al * q[j] + bprev * q1[j];
-      bb += r[j] * r[j];
-      q1[j] = q[j];
+#pragma omp parallel for schedule(static)
+    for (ch = 0; ch < NCH; ++ch) {
+      size_t k, k0 = N * ch / NCH, k1 = N * (ch + 1) / NCH;
+      double acc = 0.0;
+      for (k = k0; k < k1; ++k)
+        acc += r[2 * k] * q[2 * k] + r[2 * k + 1] * q[2 * k + 1];
+      part[ch] = acc;
     }
+    for (ch = 0; ch < NCH; ++ch) al += part[ch];
+#pragma omp parallel for schedule(static)
+    for (ch = 0; ch < NCH; ++ch) {
+      size_t k, k0 = 2 * N * ch / NCH, k1 = 2 * N * (ch + 1) / NCH;
+      double acc = 0.0;
+      for (k = k0; k < k1; ++k) {
+        r[k] -= al * q[k] + bprev * q1[k];
+        acc += r[k] * r[k];
+        q1[k] = q[k];
+      }
+      part[ch] = acc;
+    }
+    for (ch = 0; ch < NCH; ++ch) bb += part[ch];
     b = sqrt(bb);
     alpha[it] = al;
     beta[it] = b;

@@ -99,6 +99,11 @@ void or_lanczos_tridiag(int nb, int n_atoms, const double *onsite,
                         const double *blocks, const int *nbr, const double *r0,
                         int iters, double *alpha, double *beta);
 
+/* Threads of the OpenMP loops of or_hv and or_lanczos_tridiag (n <= 0:
+ * every processor); or_get_threads reports the current count. */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 /* Eigenvalues of the symmetric tridiagonal (d, e) inside (lo, hi] by
  * Sturm-sequence bisection; returns the count written to w. */
 int or_tridiag_eigs(int n, const double *d, const double *e, double lo,

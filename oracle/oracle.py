@@ -59,8 +59,19 @@ def lib():
                                       ctypes.c_double, ctypes.c_double, P,
                                       ctypes.c_int]
         L.or_tridiag_eigs.restype = ctypes.c_int
+        L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (n <= 0: every processor)."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().or_get_threads())
 
 
 def _p(a):

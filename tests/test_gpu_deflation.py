@@ -1,0 +1,97 @@
+"""Block Lanczos deflation (DeflationOn / DefTolerance, deflation_test,
+NEMO_3D/eigen/src/lanczos_blk.c:225-268): a block vector that depends on
+the ones before it is dropped and the block shrinks, instead of the run
+stopping on a rank-deficient block."""
+import numpy as np
+import pytest
+
+import _models
+from nemo3d_amd import learned, native
+from nemo3d_amd.hamiltonian import Hamiltonian
+from nemo3d_amd.solvers import eigensolve_block_lanczos
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup():
+    M = _models.gaas_box(cells=(3, 2, 2), strain_nm=0.01, seed=32)
+    ref = oracle.assemble(M.arrays())
+    w, V = np.linalg.eigh(ref.dense())
+    return M, ref, w, V
+
+
+def _distinct(w, tol=1e-8):
+    out = []
+    for x in np.sort(w):
+        if not out or x - out[-1] > tol:
+            out.append(x)
+    return np.array(out)
+
+
+def _cb_window(w):
+    """The two lowest conduction levels (distinct) and a window holding
+    exactly them."""
+    d = _distinct(w)
+    sel = d[(d > -1.0) & (d < 3.0)]
+    k = int(np.argmax(np.diff(sel)))
+    i = int(np.searchsorted(d, sel[k + 1]))
+    lo, hi = 0.5 * (d[i - 1] + d[i]), 0.5 * (d[i + 1] + d[i + 2])
+    return d[i:i + 2], (lo, hi)
+
+
+def test_rank_deficient_start_block_deflates():
+    """A start block holding one Kramers pair twice (8 columns of rank 6):
+    without deflation the block QR fails (NH_ERR_NOCONV, no eigenpairs);
+    with it the two dependent columns are dropped, the run goes on with a
+    block of 6 and finds the levels of the dense spectrum, eigenvectors
+    included."""
+    M, ref, w, V = _setup()
+    want, win = _cb_window(w)
+    i = int(np.argmin(np.abs(w - want[0])))
+    v = V[:, i]
+    pair = [v, learned.kramers_partner(v, M.nb)]
+    rng = np.random.default_rng(5)
+    rnd = rng.random((4, M.n)) + 1j * rng.random((4, M.n))
+    blk = np.array(pair + pair + list(rnd))
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    opts = dict(block_size=8, num_vb=0, num_cb=2, erange_vb=(0.0, 0.0),
+                erange_cb=win, max_iter=M.n // 8 - 10, tol=1e-8, start_iter=5,
+                skip=5, start_block=blk, return_vectors=True, stop_rule="window")
+    r0 = eigensolve_block_lanczos(H, deflation=False, **opts)
+    assert not r0["converged"] and len(r0["eigenvalues"]) == 0
+    r = eigensolve_block_lanczos(H, deflation=True, deflation_tol=1e-7, **opts)
+    print("deflated run:", r["iterations"], "blocks,", r["matvecs"], "H*v", flush=True)
+    assert r["converged"], r
+    assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
+    for e, x in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(x) - e * x) < 1e-6
+
+
+def test_deflation_during_the_recurrence():
+    """An exact eigenvector in the start block: its column of W = H Q - Q A
+    vanishes at the first step and is deflated there (the block goes from
+    8 to 7 for the rest of the run); the levels and eigenvectors still come
+    out right, and the H*v count shows the smaller block."""
+    M, ref, w, V = _setup()
+    want, win = _cb_window(w)
+    i = int(np.argmin(np.abs(w - want[1])))
+    rng = np.random.default_rng(6)
+    blk = np.concatenate([V[:, i][None, :],
+                          rng.random((7, M.n)) + 1j * rng.random((7, M.n))])
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    r = eigensolve_block_lanczos(H, block_size=8, num_vb=0, num_cb=2,
+                                 erange_vb=(0.0, 0.0), erange_cb=win,
+                                 max_iter=M.n // 8 - 10, tol=1e-8, start_iter=5,
+                                 skip=5, start_block=blk, return_vectors=True,
+                                 stop_rule="window", deflation=True,
+                                 deflation_tol=1e-7)
+    assert r["converged"], r
+    # 8 products for the first block, 7 for each block after it, in both
+    # passes, and one product per eigenvector
+    it = r["iterations"]
+    ne = len(r["eigenvalues"])
+    assert r["matvecs"] == 2 * (8 + 7 * (it - 1)) + ne, (r["matvecs"], it, ne)
+    assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
+    for e, x in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(x) - e * x) < 1e-6

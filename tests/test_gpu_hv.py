@@ -76,7 +76,8 @@ def test_hv_fp32_error_bound(case):
         y = H.hv(x, 0.0)
         yr = ref.hv(x, 0.0)
         err = np.linalg.norm(y - yr) / np.linalg.norm(yr)
-        assert 1e-10 < err < 2e-6, err
+        print(case, "FP32 rel. error %.2e" % err, flush=True)
+        assert 1e-10 < err < 3e-7, err
 
 
 def test_vector_kernels():
@@ -136,7 +137,8 @@ def test_hv_sk_fp32_error_bound(case):
     H = Hamiltonian(M, layout=native.LAYOUT_SK, precision=native.PREC_FP32)
     x = _models.random_vectors(H.n, 1, seed=13)[0]
     err = np.linalg.norm(H.hv(x) - ref.hv(x)) / np.linalg.norm(ref.hv(x))
-    assert 1e-10 < err < 2e-6, err
+    print(case, "FP32 SK rel. error %.2e" % err, flush=True)
+    assert 1e-10 < err < 3e-7, err
 
 
 @pytest.mark.parametrize("layout", ["blocks", "sk"])
@@ -162,10 +164,11 @@ def test_hV_mfma_fp32_matches_oracle(case, layout):
     for k in range(8):
         yr = ref.hv(X[k, :n], 0.6)
         err = np.linalg.norm(Y[k, :n] - yr) / np.linalg.norm(yr)
-        assert 1e-10 < err < 2e-6, (k, err)
+        print(case, layout, k, "MFMA rel. error %.2e" % err, flush=True)
+        assert 1e-10 < err < 3e-7, (k, err)
         # the single-vector FP32 product agrees to the same level
         y1 = H.hv(X[k, :n], 0.6)
-        assert np.linalg.norm(Y[k, :n] - y1) / np.linalg.norm(yr) < 2e-6
+        assert np.linalg.norm(Y[k, :n] - y1) / np.linalg.norm(yr) < 3e-7
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -186,7 +189,7 @@ def test_hv_staged_and_cached_variants_agree(case, precision):
     for k in range(3):
         yr = ref.hv(X[k], 0.2)
         err = np.linalg.norm(Ys[k] - yr) / np.linalg.norm(yr)
-        assert err <= (1e-12 if precision == "fp64" else 2e-6), err
+        assert err <= (1e-12 if precision == "fp64" else 3e-7), err
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -212,7 +215,7 @@ def test_hv_sk_quad_gather_matches_oracle(case, precision):
     bar = 1e-14 if precision == "fp64" else 1e-6
     assert np.linalg.norm(Ys - Yc) <= bar * np.linalg.norm(Yc)
     assert np.array_equal(y1, Ys[1])
-    tol = 1e-12 if precision == "fp64" else 2e-6
+    tol = 1e-12 if precision == "fp64" else 3e-7
     for k in range(3):
         yr = ref.hv(X[k], 0.3)
         assert np.linalg.norm(Ys[k] - yr) <= tol * np.linalg.norm(yr)

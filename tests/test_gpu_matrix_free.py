@@ -1,0 +1,125 @@
+"""Matrix-free layout (NH_LAYOUT_MATRIX_FREE, the reference's HamCompute =
+Recompute): the bond blocks are recomputed from the atoms' displacements on
+every product.  Checked against the oracle at the FP64 bar, against the
+stored SK layout, for exact Hermiticity, and inside the Lanczos solvers."""
+import numpy as np
+import pytest
+
+import _models
+from nemo3d_amd import native
+from nemo3d_amd.hamiltonian import Hamiltonian
+from nemo3d_amd.solvers import eigensolve_lanczos, eigensolve_mixed
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "gaas20_strained": dict(builder="gaas", cells=(3, 2, 2), strain_nm=0.02, seed=3),
+    "ingaas20_alloy_phi": dict(builder="gaas", cells=(2, 3, 2), strain_nm=0.015,
+                               seed=4, alloy=True, phi=True),
+    "gaas20_unpassivated": dict(builder="gaas", cells=(2, 2, 2), seed=5,
+                                passivate=False),
+    "si10_strained": dict(builder="si", cells=(3, 3, 3), strain_nm=0.02, seed=6),
+}
+
+
+def make(case):
+    kw = dict(CASES[case])
+    b = kw.pop("builder")
+    return _models.gaas_box(**kw) if b == "gaas" else _models.si_box(**kw)
+
+
+@pytest.mark.parametrize("staged", [1, 0])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_matrix_free_fp64_matches_oracle(case, staged):
+    """FP64 products (both neighbour-gather variants) against the oracle at
+    1e-12 relative, one and three columns, with a shift."""
+    M = make(case)
+    print("case", case, "staged", staged, flush=True)
+    ref = oracle.assemble(M.arrays())
+    H = Hamiltonian(M, layout=native.LAYOUT_MATRIX_FREE)
+    H.set(native.SET_HV_STAGED, staged)
+    X = _models.random_vectors(H.n, 3, seed=21)
+    Y = H.hV(X, 0.9)
+    for k in range(3):
+        yr = ref.hv(X[k], 0.9)
+        assert np.linalg.norm(Y[k] - yr) <= 1e-12 * np.linalg.norm(yr)
+    assert np.array_equal(H.hv(X[2], 0.9), Y[2])
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_matrix_free_equals_stored_sk(case):
+    """The recomputed records differ from the stored ones only by the
+    rounding of (d0/d)^eta (exp(eta log) against pow): products agree to
+    1e-14 relative."""
+    M = make(case)
+    Hs = Hamiltonian(M, layout=native.LAYOUT_SK)
+    Hm = Hamiltonian(M, layout=native.LAYOUT_MATRIX_FREE)
+    X = _models.random_vectors(Hs.n, 2, seed=22)
+    for x in X:
+        ys, ym = Hs.hv(x, 0.1), Hm.hv(x, 0.1)
+        assert np.linalg.norm(ys - ym) <= 1e-14 * np.linalg.norm(ys)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_matrix_free_is_exactly_hermitian(precision):
+    """Both rows of a bond get bit-identical records (the bond vector is
+    formed in the cation's frame from either end): the dense operator of a
+    small alloy box, built column by column on the device, equals its
+    conjugate transpose exactly."""
+    M = _models.gaas_box(cells=(2, 2, 1), strain_nm=0.02, seed=31, alloy=True)
+    H = Hamiltonian(M, layout=native.LAYOUT_MATRIX_FREE,
+                    precision=native.PREC_FP64 if precision == "fp64" else native.PREC_FP32)
+    n = H.n
+    print("N", n, flush=True)
+    dX = native.DeviceArray.from_host(H, np.eye(n, dtype=np.complex128))
+    dY = H.hV(dX, 0.0)
+    D = dY.download().T  # column k = H e_k
+    assert np.array_equal(D, D.conj().T)
+    ref = oracle.assemble(M.arrays()).dense()
+    tol = 1e-12 if precision == "fp64" else 3e-7
+    assert np.abs(D - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case", ["gaas20_strained", "ingaas20_alloy_phi", "si10_strained"])
+def test_matrix_free_fp32_error_bound(case):
+    """FP32 arithmetic inside each bond (integrals scaled in FP32, geometry
+    in FP64), FP64 sums over bonds: single-precision level against the
+    oracle."""
+    M = make(case)
+    ref = oracle.assemble(M.arrays())
+    H = Hamiltonian(M, layout=native.LAYOUT_MATRIX_FREE, precision=native.PREC_FP32)
+    X = _models.random_vectors(H.n, 2, seed=23)
+    for x in X:
+        yr = ref.hv(x)
+        err = np.linalg.norm(H.hv(x) - yr) / np.linalg.norm(yr)
+        print(case, "FP32 matrix-free rel. error %.2e" % err, flush=True)
+        assert 1e-10 < err < 3e-7, err
+
+
+def test_lanczos_matrix_free_against_dense():
+    """Plain FP64 Lanczos with the matrix-free products: eigenpairs of the
+    oracle's dense H to 1e-9 eV; the mixed-precision solver (FP32
+    matrix-free products to 1e-7, FP64 refinement to 1e-8) finds the same
+    levels.  Windows hold exactly the two highest valence and two lowest
+    conduction levels."""
+    from test_gpu_lanczos import distinct, tight_windows
+    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.015, seed=41, alloy=True)
+    ref = oracle.assemble(M.arrays())
+    w = distinct(np.linalg.eigvalsh(ref.dense()))
+    vb, cb, wv, wc = tight_windows(w, 2, 2)
+    want = np.sort(np.concatenate([vb, cb]))
+    H = Hamiltonian(M, layout=native.LAYOUT_MATRIX_FREE)
+    r = eigensolve_lanczos(H, num_vb=2, num_cb=2, erange_vb=wv, erange_cb=wc,
+                           max_iter=6000, tol=1e-10, start_iter=100, skip=25,
+                           return_vectors=True, capacity=4)
+    assert r["converged"]
+    assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
+    for e, v in zip(r["eigenvalues"], r["vectors"]):
+        assert np.linalg.norm(ref.hv(v) - e * v) / np.linalg.norm(v) < 1e-8 * 40
+    rm = eigensolve_mixed(H, num_vb=2, num_cb=2, erange_vb=wv, erange_cb=wc,
+                          tol=1e-8, tol32=1e-7, hnorm=40.0, max_iter=6000,
+                          start_iter=100, skip=25, refine_depth=12,
+                          refine_rounds=8)
+    assert rm["converged"], rm
+    assert np.abs(np.sort(rm["eigenvalues"]) - want).max() < 1e-9

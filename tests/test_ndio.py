@@ -55,3 +55,18 @@ def test_rewrite_reference_files_bytewise(name, tmp_path, reference_root):
     out = str(tmp_path / name)
     ndio.write(out, nd.fmt, nd.dims, nd.user, nd.data)
     assert open(out, "rb").read() == open(src, "rb").read()
+
+
+def test_tridiagonal_checkpoint_layout(tmp_path):
+    """The checkpoint layout of saveTridiagonalMatrix (lanczos.c:1692-1714):
+    int32 iterations, int32 processors, alpha/beta interleaved."""
+    import struct
+    a = np.array([1.5, -2.0, 3.25])
+    b = np.array([0.5, 0.75, 0.125])
+    p = tmp_path / "t"
+    ndio.write_tridiagonal(p, a, b, processors=16)
+    raw = open(p, "rb").read()
+    assert struct.unpack("<ii", raw[:8]) == (3, 16)
+    assert struct.unpack("<6d", raw[8:]) == (1.5, 0.5, -2.0, 0.75, 3.25, 0.125)
+    a2, b2, procs = ndio.read_tridiagonal(p)
+    assert procs == 16 and np.array_equal(a2, a) and np.array_equal(b2, b)

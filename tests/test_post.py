@@ -1,0 +1,67 @@
+"""Post-processing: the reference's local strain tensor and probability
+densities (nemo3d_amd.post)."""
+import numpy as np
+
+from nemo3d_amd import materials, model, ndio, post, structure, vff
+
+
+def _box(cells=(4, 4, 4)):
+    a = 0.56532
+    s = structure.binary_box(cells, "Ga", "As", (a, a, a))
+    return s, model.TBModel(s, ["GaAs"], materials.BANDS_20_SPIN)
+
+
+def test_block_labels_cover_the_four_bonds():
+    """Every atom's four block labels (compute_strain_tensor's
+    map_mb_to_m) name four distinct neighbour slots."""
+    mb, mm = post._block_label_maps()
+    for m in range(8):
+        assert sorted(mm[m, mb[m]]) == [0, 1, 2, 3]
+
+
+def test_strain_tensor_of_a_homogeneous_deformation():
+    """Displacements u = E r of the ideal sites (E a general small 3x3
+    matrix): every atom with four neighbours reports A A0^-1 - 1 = E,
+    i.e. exx, eyy, ezz, exy, exz, eyz = E00, E11, E22, E01, E02, E12
+    (whatever edges are chosen, for a homogeneous deformation); atoms
+    without four neighbours report zeros."""
+    s, M = _box()
+    E = np.random.default_rng(3).normal(0, 0.01, (3, 3))
+    s.disp = s.positions(strained=False) @ E.T
+    _, _, a0 = vff.keating_tables(M)
+    eps, ok = post.strain_tensor(s, M.species, a0)
+    want = np.array([E[0, 0], E[1, 1], E[2, 2], E[0, 1], E[0, 2], E[1, 2]])
+    assert ok.sum() > 0 and (~ok).sum() > 0
+    assert np.abs(eps[ok] - want).max() < 1e-12
+    assert not eps[~ok].any()
+
+
+def test_strain_tensor_against_lattice_mismatch():
+    """Ideal GaAs sites measured against the InAs lattice constant: the
+    strain is (a_GaAs / a_InAs - 1) on the diagonal, zero off it."""
+    s, M = _box()
+    _, _, a0 = vff.keating_tables(M)
+    a_inas = 0.60583
+    eps, ok = post.strain_tensor(s, M.species, np.where(a0 > 0, a_inas, 0.0))
+    d = 0.56532 / a_inas - 1.0
+    assert np.abs(eps[ok][:, :3] - d).max() < 1e-13
+    assert np.abs(eps[ok][:, 3:]).max() < 1e-13
+
+
+def test_psi_squared_and_files(tmp_path):
+    s, M = _box((2, 2, 2))
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=M.n) + 1j * rng.normal(size=M.n)
+    p2 = post.psi_squared(v, M.nb)
+    assert p2.max() == 1.0 and p2.shape == (M.n_atoms,)
+    cs = post.cell_sums(p2, s)
+    assert abs(cs.sum() - p2.sum()) < 1e-12
+    names = post.write_psi_sqr(str(tmp_path / "dot"), [v], M.nb, "Bands_20_sp3d5ss_spin", s)
+    d = ndio.read(names[0])
+    assert np.allclose(np.asarray(d.data["psi2"]).ravel(), p2)
+    _, _, a0 = vff.keating_tables(M)
+    eps, ok = post.strain_tensor(s, M.species, a0)
+    post.write_strain3d(tmp_path / "strain3d", tmp_path / "strain3d_cation", s, eps, ok)
+    rows = np.loadtxt(tmp_path / "strain3d")
+    assert rows.shape == (M.n_atoms, 9)
+    assert np.loadtxt(tmp_path / "strain3d_cation").shape[0] == int(s.is_cation.sum())
