@@ -93,6 +93,11 @@ enum { NH_SET_HV_STAGED = 1 };
  * halved down to 8 until the ring takes at most half the free memory.
  * The eigenvectors agree to rounding whatever the length. */
 enum { NH_SET_RING_STEPS = 2 };
+/* NH_SET_BLOCK_HV: products of several vectors (block H·V) in the SK and
+ * matrix-free layouts: -1 (default) by measurement, 0 one launch per
+ * column, 1 all columns in one launch with the bond records read once,
+ * 2 the matrix-core kernel (FP32 products, 8 columns). */
+enum { NH_SET_BLOCK_HV = 3 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 
 /* Assemble H on the device in the given layout and precision.  Replaces

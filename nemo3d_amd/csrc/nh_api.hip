@@ -322,6 +322,11 @@ nh_status nh_context_set(nh_context *ctx, int what, int value) {
                "NH_SET_HV_STAGED: value %d not in -1..1", value);
       ctx->hv_staged = value;
       return NH_OK;
+    case NH_SET_BLOCK_HV:
+      NH_CHECK(value >= -1 && value <= 2, NH_ERR_ARG,
+               "NH_SET_BLOCK_HV: value %d not in -1..2", value);
+      ctx->block_hv = value;
+      return NH_OK;
     case NH_SET_RING_STEPS:
       NH_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64 ||
                    value == 128,

@@ -76,6 +76,76 @@ __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
   }
 }
 
+// Block H·V in the SK-compressed and matrix-free layouts: the columns of
+// the block one after another inside each workgroup, the bond records,
+// neighbour indices and onsite records read once for all of them
+// (Hmatmult_spds_col_stored, Hmult_stored.c:2359-2474, re-reads H for every
+// column).  Mapping and the per-column steps as k_hv_sk with the quad LDS
+// gather.
+template <int NSP, typename R, bool CATION, typename SRC>
+__device__ __forceinline__ void sk_columns(int atom, int slot, bool active,
+                                           const SRC &src,
+                                           const int32_t *__restrict__ nbr,
+                                           const double *__restrict__ onsite,
+                                           double shift, const double2 *__restrict__ X,
+                                           double2 *__restrict__ Y, int64_t ld,
+                                           int ncol, SkQuadStage<NSP, double2> &L) {
+  constexpr int NB = NSP * 10;
+  constexpr int NO = (NB + 3) / 4;
+  const int j = active ? nbr[atom * 4 + slot] : -1;
+  R rv[24];
+  if (j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
+  SkOwnRegs<NSP> own;
+  auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L.buf);
+  const int la = threadIdx.x / 4;
+  for (int col = 0; col < ncol; ++col) {
+    const double2 *xc = X + (int64_t)col * ld;
+    double2 *yc = Y + (int64_t)col * ld;
+    double own_re[NO], own_im[NO];
+    sk_bonds_rec<NSP, R, CATION, double2>(j, slot, rv, xc, own_re, own_im, L);
+    if (active)
+      sk_load_own<NSP>(slot, xc + (int64_t)atom * NB,
+                       onsite + (size_t)atom * kOnsiteStride, own);
+    __syncthreads();  // every lane has read its staged neighbour entries
+    if (active) sk_store_own<NSP>(la, slot, own, S);
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < NO; ++i) {
+        const int r = 4 * i + slot;
+        if (r >= NB) break;
+        double re = own_re[i], im = own_im[i];
+        onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la],
+                        r / 10, r % 10, shift, re, im);
+        yc[(int64_t)atom * NB + r] = make_double2(re, im);
+      }
+    }
+    __syncthreads();  // the epilogue has read the staging before it is reused
+  }
+}
+
+template <int NSP, typename R, typename SRC>
+__global__ __launch_bounds__(kSkThreads) void k_hV_sk(
+    int nc, const int32_t *__restrict__ cations, int na,
+    const int32_t *__restrict__ anions, int ppx,
+    const double *__restrict__ onsite, const SRC src,
+    const int32_t *__restrict__ nbr, double shift,
+    const double2 *__restrict__ X, double2 *__restrict__ Y, int64_t ld, int ncol) {
+  const SkBlock wb = sk_block_of(blockIdx.x, ppx);
+  const bool cat = wb.cat;
+  const int li = wb.bl * kSkAtomsPerBlock + threadIdx.x / 4;
+  const int slot = threadIdx.x & 3;
+  const bool active = li < (cat ? nc : na);
+  const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
+  __shared__ SkQuadStage<NSP, double2> stage;
+  if (cat)
+    sk_columns<NSP, R, true>(atom, slot, active, src, nbr, onsite, shift, X, Y,
+                             ld, ncol, stage);
+  else
+    sk_columns<NSP, R, false>(atom, slot, active, src, nbr, onsite, shift, X, Y,
+                              ld, ncol, stage);
+}
+
 template <int NSP, typename TB, typename TA, bool STAGED>
 __global__ __launch_bounds__(kThreads) void k_hv_blocks(
     int A, const double *__restrict__ onsite, const TB *__restrict__ blocks,
@@ -115,7 +185,38 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
            "H·v: Hamiltonian not built in a supported layout");
   NH_CHECK(ncol >= 1 && ncol <= 65535 && ld >= c->n, NH_ERR_ARG,
            "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
-  if (hV_mfma_supported(c, ncol)) return launch_hV_mfma(c, shift, x, y, ld);
+  // block products: the matrix-core kernel (FP32, 8 columns) or all the
+  // columns in one SK / matrix-free launch
+  const int mode = c->block_hv;
+  if ((mode == 2 || mode == -1) && hV_mfma_supported(c, ncol))
+    return launch_hV_mfma(c, shift, x, y, ld);
+  if (sk_like(c) && ncol > 1 && mode != 0) {
+    const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
+    const double2 *xd = reinterpret_cast<const double2 *>(x);
+    double2 *yd = reinterpret_cast<double2 *>(y);
+#define HVK(NSP, R, SRC)                                                      \
+  hipLaunchKernelGGL((k_hV_sk<NSP, R, decltype(SRC)>), dim3(16 * ppx),        \
+                     dim3(kSkThreads), 0, c->stream, c->n_cations,            \
+                     c->cations, c->n_anions, c->anions, ppx, c->onsite, SRC, \
+                     c->nbr, shift, xd, yd, ld, ncol)
+#define HVP(R, SRC)                                                          \
+  do {                                                                       \
+    if (c->nb == 20) HVK(2, R, SRC);                                         \
+    else HVK(1, R, SRC);                                                     \
+  } while (0)
+    if (c->layout == NH_LAYOUT_MATRIX_FREE) {
+      if (c->precision == NH_PREC_FP64) HVP(double, mf_src<double>(c));
+      else HVP(float, mf_src<float>(c));
+    } else if (c->precision == NH_PREC_FP64) {
+      HVP(double, (SkSrc<double>{c->sk64, c->bond_of}));
+    } else {
+      HVP(float, (SkSrc<float>{c->sk32, c->bond_of}));
+    }
+#undef HVP
+#undef HVK
+    NH_HIP(hipGetLastError());
+    return NH_OK;
+  }
   if (sk_like(c)) {
     const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
     const dim3 g(16 * ppx, ncol);

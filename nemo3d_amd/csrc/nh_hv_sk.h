@@ -473,4 +473,40 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
     }
 }
 
+// The same with the bond record already in registers (rv; j < 0: no bond)
+// and the quad LDS gather of XT entries: the block products of several
+// vectors reuse one record.
+template <int NSP, typename R, bool CATION, typename XT>
+__device__ __forceinline__ void sk_bonds_rec(int j, int slot, const R (&rv)[24],
+                                             const XT *__restrict__ xc,
+                                             double (&own_re)[(NSP * 10 + 3) / 4],
+                                             double (&own_im)[(NSP * 10 + 3) / 4],
+                                             SkQuadStage<NSP, XT> &L) {
+  R br[NSP][10], bi[NSP][10];
+#pragma unroll
+  for (int s = 0; s < NSP; ++s)
+#pragma unroll
+    for (int c = 0; c < 10; ++c) br[s][c] = bi[s][c] = R(0);
+  R xr[NSP][10], xi[NSP][10];
+  sk_quad_gather<NSP, R, XT>(j, slot, xc, L, xr, xi);
+  if (j >= 0)
+    BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3, xr,
+                                                xi, br, bi);
+#pragma unroll
+  for (int s = 0; s < NSP; ++s)
+#pragma unroll
+    for (int c = 0; c < 10; ++c) {
+      const int r = s * 10 + c;
+      double re = double(br[s][c]), im = double(bi[s][c]);
+      re += quad_perm_f64<kQuadXor1>(re);
+      im += quad_perm_f64<kQuadXor1>(im);
+      re += quad_perm_f64<kQuadXor2>(re);
+      im += quad_perm_f64<kQuadXor2>(im);
+      if ((r & 3) == slot) {
+        own_re[r >> 2] = re;
+        own_im[r >> 2] = im;
+      }
+    }
+}
+
 }  // namespace nh

@@ -87,6 +87,7 @@ struct nh_context {
   // kernel, so off by default)
   int hv_staged = -1;  // NH_SET_HV_STAGED (-1: by layout, see use_staged)
   int ring_steps = 0;  // NH_SET_RING_STEPS (0: by free memory)
+  int block_hv = -1;   // NH_SET_BLOCK_HV (-1: by measurement)
 
   // assembled Hamiltonian
   int layout = -1;

@@ -18,6 +18,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
 SET_HV_STAGED = 1
 SET_RING_STEPS = 2
+SET_BLOCK_HV = 3
 
 _STATUS = {0: "NH_OK", 1: "NH_ERR_ARG", 2: "NH_ERR_HIP", 3: "NH_ERR_STATE",
            4: "NH_ERR_NOMEM", 5: "NH_ERR_NOCONV"}
