@@ -49,7 +49,8 @@ os.environ.setdefault("NEMOHIP_PROGRESS", "1")  # progress lines on stderr
 
 from nemo3d_amd import native, vff, warmstart  # noqa: E402
 from nemo3d_amd.hamiltonian import Hamiltonian  # noqa: E402
-from nemo3d_amd.solvers import eigensolve_lanczos, eigensolve_mixed  # noqa: E402
+from nemo3d_amd.solvers import (eigensolve_lanczos, eigensolve_mixed,  # noqa: E402
+                                eigensolve_mixed_block, eigensolve_tracemin)
 
 HBM_PEAK_GBS = 8000.0
 WEIGHTS = os.path.join(HERE, "nemo3d_amd", "data", "warmstart.npz")
@@ -121,6 +122,70 @@ def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False):
     return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
+def layout_probe(M, steps=600):
+    """Mean launch time (HIP events) of the fused Lanczos H*v kernel of the
+    eigenvalue pass for the stored-SK and the matrix-free layouts, FP64 and
+    FP32 products, each from a short Lanczos run of `steps` steps on the
+    same structure (outside the timed region); the matrix-free layout
+    recomputes every bond record per product, the SK layout reads it."""
+    out = {}
+    opts = dict(num_vb=0, num_cb=1, erange_cb=(1e3, 1e3 + 1), max_iter=steps,
+                start_iter=10 ** 7, want_vectors=False)
+    for name, lay in (("sk", native.LAYOUT_SK),
+                      ("matrix_free", native.LAYOUT_MATRIX_FREE)):
+        Hx = Hamiltonian(M, layout=lay)
+        for pn, prec in (("fp64", native.PREC_FP64), ("fp32", native.PREC_FP32)):
+            Hx.set_precision(prec)
+            eigensolve_lanczos(Hx, **dict(opts, max_iter=50))
+            r = eigensolve_lanczos(Hx, **opts)
+            out["%s_%s_hv_ms" % (name, pn)] = r["hv_ms"]
+            out["%s_%s_step_ms" % (name, pn)] = 1e3 * r["seconds"] / r["iterations"]
+        Hx.close()
+    return out
+
+
+def learned_routes(H, net, inputs, windows, hnorm, max_block_steps=3000):
+    """The learned block (8 columns) as the start of the block solvers, one
+    run each outside the timed region: FP32 block Lanczos (block H*V of 8
+    columns) to 3e-8 then the FP64 refinement to 1e-8, and PQ-TRACEMIN on
+    (H - sigma)^2 with sigma at mid-gap (the 8 levels nearest it are the 4 + 4
+    of the windows), FP64, capped at 40 outer steps."""
+    out = {}
+    t0 = time.time()
+    blk = warmstart.start_block(net, H, inputs=inputs)
+    t_blk = time.time() - t0
+    t0 = time.time()
+    r = eigensolve_mixed_block(H, block_size=8, num_vb=4, num_cb=4, tol=1e-8,
+                               tol32=TOL32, max_iter32=max_block_steps,
+                               start_block=blk, stop_rule="window", stage2="refine",
+                               hnorm=hnorm, refine_depth=12, refine_rounds=4,
+                               deflation=True, **windows)
+    out["block_lanczos_fp32_then_refine"] = {
+        "seconds": time.time() - t0 + t_blk, "cnn_start_s": t_blk,
+        "converged": bool(r["converged"]),
+        "block_steps": int(r["stage32"]["iterations"]),
+        "matvecs": int(r["matvecs"]),
+        "block_hv_ms_per_8_columns": r["stage32"]["hv_ms"],
+        "eigenvalues_eV": np.round(r["eigenvalues"], 9).tolist(),
+        "rel_residuals": (np.asarray(r["residuals"]) / hnorm).tolist()}
+    log("learned block -> block Lanczos: %.2f s, %d block steps, converged %s" % (
+        out["block_lanczos_fp32_then_refine"]["seconds"], r["stage32"]["iterations"],
+        r["converged"]))
+    sigma = 0.5 * (windows["erange_vb"][1] + windows["erange_cb"][0])
+    t0 = time.time()
+    r = eigensolve_tracemin(H, num_eig=8, ref_eig=sigma, block_size=8, tol=1e-8,
+                            hnorm=hnorm, max_outer=40, start_block=blk)
+    out["pq_tracemin_fp64"] = {
+        "seconds": time.time() - t0 + t_blk, "converged": bool(r["converged"]),
+        "outer_steps": int(r["iterations"]), "matvecs": int(r["matvecs"]),
+        "sigma_eV": sigma,
+        "eigenvalues_eV": np.round(r["eigenvalues"], 9).tolist(),
+        "rel_residuals": (np.asarray(r["residuals"]) / hnorm).tolist()}
+    log("learned block -> PQ-TRACEMIN: %.2f s, %d outer steps, converged %s" % (
+        out["pq_tracemin_fp64"]["seconds"], r["iterations"], r["converged"]))
+    return out
+
+
 def host_threads():
     """Threads the host offers this process (OMP_NUM_THREADS when set, else
     the CPU affinity)."""
@@ -138,6 +203,17 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def mf_lanczos_bytes(M, fp32_copy=False):
+    """Algorithmic bytes of one fused Lanczos H*v launch in the matrix-free
+    layout: onsite record (18 doubles), displacement + species (4 doubles),
+    the packed ideal bond vectors and the neighbour indices (4 + 4 int32)
+    and the list entry per atom, x read once (FP64), y written once, and
+    with FP32 products the FP32 copy of x (read once)."""
+    A = M.n_atoms
+    mat = A * (18 * 8 + 4 * 8 + 4 * 4 * 2 + 4)
+    return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
 def cpu_leg(M, matvecs, checks, sample_iters=2):
@@ -237,6 +313,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iter", type=int, default=30000)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dot786k")
+    ap.add_argument("--layout", choices=["sk", "mf"], default="sk",
+                    help="Hamiltonian layout of the solves: stored SK records "
+                         "or matrix-free (records recomputed per product)")
+    ap.add_argument("--no-layout-probe", action="store_true")
+    ap.add_argument("--no-learned-routes", action="store_true",
+                    help="skip the learned-block block-Lanczos and TRACEMIN runs")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     hnorm, windows = wl["hnorm"], wl["windows"]
@@ -253,7 +335,8 @@ def main():
     vinfo = vff.relax(M)
     setup["vff_s"] = time.time() - t1
     t1 = time.time()
-    H = Hamiltonian(M, layout=native.LAYOUT_SK, precision=native.PREC_FP64)
+    layout = native.LAYOUT_SK if args.layout == "sk" else native.LAYOUT_MATRIX_FREE
+    H = Hamiltonian(M, layout=layout, precision=native.PREC_FP64)
     H.set_precision(native.PREC_FP32)  # FP32 copies made outside the timing
     H.set_precision(native.PREC_FP64)
     setup["assembly_s"] = time.time() - t1
@@ -348,16 +431,24 @@ def main():
             cfg["eig_diff_vs_plain_eV"] = d.tolist()
             cfg["speedup_vs_plain"] = cfg["plain_fp64_s"] / cfg["mixed_s"]
 
+    mf = args.layout == "mf"
+    lname = "matrix-free" if mf else "SK"
+
+    def lz_bytes(fp32):
+        if mf:
+            return mf_lanczos_bytes(M, fp32_copy=fp32)
+        return sk_lanczos_bytes(M, rec_bytes=4, fp32_copy=True) if fp32 else sk_lanczos_bytes(M)
+
     if mixed is not None:
-        nbytes = sk_lanczos_bytes(M, rec_bytes=4, fp32_copy=True)
+        nbytes = lz_bytes(True)
         kms = mixed["stage32"]["hv_ms"]
-        kname = "k_lz_hv_sk<2,float> (fused Lanczos H*v, SK FP32 products)"
-        tr_key = "k_lz_hv_sk_fp32"
+        kname = "k_lz_hv_sk<2,float> (fused Lanczos H*v, %s FP32 products)" % lname
+        tr_key = None if mf else "k_lz_hv_sk_fp32"
     else:
-        nbytes = sk_lanczos_bytes(M)
+        nbytes = lz_bytes(False)
         kms = plain["hv_ms"]
-        kname = "k_lz_hv_sk<2,double> (fused Lanczos H*v, SK FP64)"
-        tr_key = "k_lz_hv_sk"
+        kname = "k_lz_hv_sk<2,double> (fused Lanczos H*v, %s FP64)" % lname
+        tr_key = None if mf else "k_lz_hv_sk"
     achieved = nbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     line = {
         "metric": "s to 8 eigenpairs @1e-8 resid (mixed+learned vs FP64 plain); H·v HBM GB/s",
@@ -381,23 +472,29 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             # measured (PMC) on the default workload only
-            "traffic": traffic_from_profiles(tr_key) if args.workload == "dot786k" else None,
+            "traffic": traffic_from_profiles(tr_key)
+            if (args.workload == "dot786k" and tr_key) else None,
             "bytes_per_launch": nbytes,
             "launch_ms": kms,
         },
     }
     if plain is not None:
-        pb = sk_lanczos_bytes(M)
+        pb = lz_bytes(False)
         line["roofline_plain"] = {
-            "kernel": "k_lz_hv_sk<2,double> (fused Lanczos H*v, SK FP64)",
+            "kernel": "k_lz_hv_sk<2,double> (fused Lanczos H*v, %s FP64)" % lname,
             "achieved": pb / (plain["hv_ms"] * 1e-3) / 1e9 if plain["hv_ms"] > 0 else 0.0,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_launch": pb,
             "launch_ms": plain["hv_ms"],
             "traffic": traffic_from_profiles("k_lz_hv_sk")
-            if args.workload == "dot786k" else None}
+            if (args.workload == "dot786k" and not mf) else None}
         line["roofline_plain"]["frac"] = line["roofline_plain"]["achieved"] / HBM_PEAK_GBS
     if args.workload != "dot786k":
         cfg["lanczos_vector_memory"] = lanczos_memory(M.n)
+    cfg["layout"] = {"sk": "stored SK records", "mf": "matrix-free"}[args.layout]
+    if not args.no_learned_routes and net is not None:
+        cfg["learned_block_routes"] = learned_routes(H, net, inputs, windows, hnorm)
+    if not args.no_layout_probe:
+        cfg["layout_comparison"] = layout_probe(M)
     if not args.no_cpu_baseline and plain is not None and args.workload == "dot786k":
         checks = {"plain": (plain["eigenvalues"],
                             pvec.download()[:len(plain["eigenvalues"])])}

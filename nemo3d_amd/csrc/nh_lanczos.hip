@@ -954,9 +954,27 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
     NH_HIP(hipMemcpyAsync(beta_, o->restore_beta, (size_t)m * sizeof(double),
                           hipMemcpyHostToDevice, c->stream));
     NH_TRY(fetch(m));
-    NH_TRY(analyse(m, o));
-    done = window_done(0, o->num_eig_vb, o->stop_rule) &&
-           window_done(1, o->num_eig_cb, o->stop_rule);
+    // the analyses of the original run, at the same check points (which
+    // converged values were recorded depends on where they were seen)
+    int mm = 0, sk = 0;
+    while (mm < m) {
+      ++mm;
+      ++sk;
+      if (o->conv_check_start < mm && o->conv_check_skip <= sk) {
+        sk = 0;
+        NH_TRY(analyse(mm, o));
+        done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+               window_done(1, o->num_eig_cb, o->stop_rule);
+        if (done) break;
+      }
+    }
+    if (!done) {
+      NH_TRY(analyse(m, o));
+      done = window_done(0, o->num_eig_vb, o->stop_rule) &&
+             window_done(1, o->num_eig_cb, o->stop_rule);
+    } else {
+      m = mm;
+    }
     pending = m;
   }
   while (o->restore_len <= 0 && m + 1 < o->max_iter) {

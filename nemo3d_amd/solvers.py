@@ -106,7 +106,7 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
                              seed=1, start_block=None, want_vectors=True,
                              capacity=64, return_vectors=False,
                              stop_rule="window", deflation=False,
-                             deflation_tol=1e-7):
+                             deflation_tol=1e-7, vectors_out=None):
     """Block Lanczos (eigensolve_blk_lanczos, NEMO_3D/eigen/src/
     lanczos_blk.c:2145-...) with BL_size = block_size.  num_vb / num_cb
     count distinct levels; stop_rule as in eigensolve_lanczos.  Without a
@@ -137,6 +137,10 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
     ev = np.zeros(capacity)
     rs = np.full(capacity, -1.0)
     vec = np.zeros((capacity, H.n), dtype=np.complex128) if return_vectors else None
+    if vectors_out is not None:  # device array (capacity, N): vectors stay on the GPU
+        assert vectors_out.shape == (capacity, H.n) and vec is None
+        vec = vectors_out
+        o.want_vectors = 1
     r = native.NHEigResult()
     r.eigenvalues = native.ptr(ev)
     r.residuals = native.ptr(rs)
@@ -151,7 +155,7 @@ def eigensolve_block_lanczos(H, block_size=8, num_vb=4, num_cb=4,
            "seconds": r.seconds, "converged": st == native.NH_OK,
            "hv_ms": r.hv_ms, "hv_timed": r.hv_timed}
     if vec is not None:
-        out["vectors"] = vec[:n]
+        out["vectors"] = vec if vectors_out is not None else vec[:n]
     return out
 
 
@@ -350,25 +354,79 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
 
 
 def eigensolve_mixed_block(H, block_size=8, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
-                     erange_cb=(1.4, 2.0), tol=1e-8, tol32=1e-6,
-                     max_iter32=2000, max_iter64=400, seed=1, start_block=None,
-                     stop_rule="window", return_vectors=False, capacity=64):
+                           erange_cb=(1.4, 2.0), tol=1e-8, tol32=1e-6,
+                           max_iter32=2000, max_iter64=400, seed=1, start_block=None,
+                           stop_rule="window", return_vectors=False, capacity=64,
+                           stage2="block", hnorm=40.0, refine_depth=12,
+                           refine_rounds=4, deflation=False):
     """Mixed-precision block Lanczos: the bulk iterations with FP32 H·V
     (FP32 values, FP32 products inside each block; on the matrix cores for
     a block of 8) and FP64 recurrences, dots and Ritz solves, to an
-    FP32-level tolerance `tol32`; then FP64 block Lanczos started from
-    those Ritz vectors to `tol`, with FP64 Rayleigh quotients and residuals.
+    FP32-level tolerance `tol32`; then, stage2 = "block", FP64 block
+    Lanczos started from those Ritz vectors to `tol`, with FP64 Rayleigh
+    quotients and residuals, or, stage2 = "refine", the FP64 residual
+    refinement of eigensolve_mixed (refine_eigenpairs per side of the gap,
+    ||H x - e x|| <= tol * hnorm) on the device-resident Ritz vectors.
     start_block: optional (block_size, N) start, e.g. the learned one."""
     from . import native as nt
     prec0 = H.precision
     t_all = 0.0
+    if stage2 == "refine":
+        cap = num_vb + num_cb
+        dv = nt.DeviceArray(H, (cap, H.n))
+        try:
+            H.set_precision(nt.PREC_FP32)
+            r1 = eigensolve_block_lanczos(
+                H, block_size=block_size, num_vb=num_vb, num_cb=num_cb,
+                erange_vb=erange_vb, erange_cb=erange_cb, max_iter=max_iter32,
+                tol=tol32, start_iter=4, skip=4, seed=seed, start_block=start_block,
+                capacity=cap, stop_rule=stop_rule, deflation=deflation,
+                vectors_out=dv)
+            H.set_precision(nt.PREC_FP64)
+            n = len(r1["eigenvalues"])
+            ev32 = np.asarray(r1["eigenvalues"])
+            split = int(np.searchsorted(ev32, 0.5 * (erange_vb[1] + erange_cb[0])))
+            rf = {"eigenvalues": np.zeros(n), "residuals": np.zeros(n), "rounds": 0,
+                  "matvecs": 0, "seconds": 0.0, "converged": n > 0}
+            for a, b in ((0, split), (split, n)):
+                if b <= a:
+                    continue
+                g = refine_eigenpairs(H, dv, ev32[a:b], tol=tol, hnorm=hnorm,
+                                      depth=refine_depth, max_rounds=refine_rounds,
+                                      row0=a)
+                rf["eigenvalues"][a:b] = g["eigenvalues"]
+                rf["residuals"][a:b] = g["residuals"]
+                rf["rounds"] = max(rf["rounds"], g["rounds"])
+                rf["matvecs"] += g["matvecs"]
+                rf["seconds"] += g["seconds"]
+                rf["converged"] = rf["converged"] and g["converged"]
+        finally:
+            if prec0 is not None:
+                H.set_precision(prec0)
+        order = np.argsort(rf["eigenvalues"])
+        out = {"eigenvalues": rf["eigenvalues"][order],
+               "residuals": rf["residuals"][order],
+               "iterations": r1["iterations"],
+               "matvecs": r1["matvecs"] + rf["matvecs"],
+               "seconds": r1["seconds"] + rf["seconds"],
+               "converged": bool(r1["converged"] and rf["converged"]),
+               "hv_ms": r1["hv_ms"], "hv_timed": r1["hv_timed"],
+               "stage32": {k: r1[k] for k in ("iterations", "matvecs", "seconds",
+                                              "converged", "hv_ms", "hv_timed",
+                                              "eigenvalues")},
+               "refine": {k: rf[k] for k in ("rounds", "matvecs", "seconds",
+                                             "converged")}}
+        if return_vectors:
+            out["vectors"] = dv.download()[:len(order)][order] if n else np.zeros((0, H.n))
+        return out
     try:
         H.set_precision(nt.PREC_FP32)
         r1 = eigensolve_block_lanczos(
             H, block_size=block_size, num_vb=num_vb, num_cb=num_cb,
             erange_vb=erange_vb, erange_cb=erange_cb, max_iter=max_iter32,
             tol=tol32, start_iter=4, skip=4, seed=seed, start_block=start_block,
-            return_vectors=True, capacity=capacity, stop_rule=stop_rule)
+            return_vectors=True, capacity=capacity, stop_rule=stop_rule,
+            deflation=deflation)
         t_all += r1["seconds"]
         V = r1["vectors"]
         if len(V) < block_size:
@@ -383,7 +441,7 @@ def eigensolve_mixed_block(H, block_size=8, num_vb=4, num_cb=4, erange_vb=(-0.6,
             erange_vb=erange_vb, erange_cb=erange_cb, max_iter=max_iter64,
             tol=tol, start_iter=1, skip=1, start_block=V,
             return_vectors=return_vectors, capacity=capacity,
-            stop_rule=stop_rule)
+            stop_rule=stop_rule, deflation=deflation)
         t_all += r2["seconds"]
     finally:
         if prec0 is not None:

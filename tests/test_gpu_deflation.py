@@ -55,8 +55,10 @@ def test_rank_deficient_start_block_deflates():
     rnd = rng.random((4, M.n)) + 1j * rng.random((4, M.n))
     blk = np.array(pair + pair + list(rnd))
     H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    # the block shrinks to 4 (2 copies dropped at the start, the pair's own
+    # directions at the first step): up to N / 4 block steps
     opts = dict(block_size=8, num_vb=0, num_cb=2, erange_vb=(0.0, 0.0),
-                erange_cb=win, max_iter=M.n // 8 - 10, tol=1e-8, start_iter=5,
+                erange_cb=win, max_iter=M.n // 4 - 10, tol=1e-8, start_iter=5,
                 skip=5, start_block=blk, return_vectors=True, stop_rule="window")
     r0 = eigensolve_block_lanczos(H, deflation=False, **opts)
     assert not r0["converged"] and len(r0["eigenvalues"]) == 0

@@ -199,8 +199,7 @@ def test_hv_sk_quad_gather_matches_oracle(case, precision):
     entries: products equal to the direct gather's to FP64 / FP32 rounding
     (the compiler contracts the block's multiply-adds differently around
     the staged loads), oracle to the FP64 / FP32 bar, single and 3-column
-    products (the single product equal to the block's column bit for
-    bit)."""
+    products (equal to rounding)."""
     M = make(case)
     print("case", case, precision, flush=True)
     ref = oracle.assemble(M.arrays())
@@ -214,7 +213,9 @@ def test_hv_sk_quad_gather_matches_oracle(case, precision):
     Yc = H.hV(X, 0.3)
     bar = 1e-14 if precision == "fp64" else 1e-6
     assert np.linalg.norm(Ys - Yc) <= bar * np.linalg.norm(Yc)
-    assert np.array_equal(y1, Ys[1])
+    # the 3 columns go through the block kernel (records read once): the
+    # same products, contracted differently by the compiler
+    assert np.linalg.norm(y1 - Ys[1]) <= bar * np.linalg.norm(y1)
     tol = 1e-12 if precision == "fp64" else 3e-7
     for k in range(3):
         yr = ref.hv(X[k], 0.3)

@@ -44,7 +44,8 @@ def test_matrix_free_fp64_matches_oracle(case, staged):
     for k in range(3):
         yr = ref.hv(X[k], 0.9)
         assert np.linalg.norm(Y[k] - yr) <= 1e-12 * np.linalg.norm(yr)
-    assert np.array_equal(H.hv(X[2], 0.9), Y[2])
+    y1 = H.hv(X[2], 0.9)  # single product: the same to rounding
+    assert np.linalg.norm(y1 - Y[2]) <= 1e-14 * np.linalg.norm(y1)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))

@@ -21,11 +21,10 @@ def mem_info(H):
 
 def test_allocation_takes_back_idle_workspace():
     """After a refinement (a 104-vector Krylov basis, ~0.8 GB, kept idle in
-    the context's pool), an nh_device_alloc of the device's free memory
-    plus the idle pool minus 128 MB succeeds: the allocator frees the idle
-    blocks when the device is full.  A second context's allocation takes
-    back the first context's idle blocks as well; the solver still works
-    afterwards."""
+    the context's pool), an allocation the device cannot serve first
+    returns the idle blocks of every context on the device, then fails
+    cleanly (NH_ERR_NOMEM); an allocation of the device's free memory plus
+    the idle pool minus 128 MB succeeds; the solver works afterwards."""
     M = _models.gaas_box(cells=(15, 15, 14), seed=3)
     H = Hamiltonian(M, layout=native.LAYOUT_SK)
     print("N =", H.n, flush=True)
@@ -34,18 +33,18 @@ def test_allocation_takes_back_idle_workspace():
     fr, idle = mem_info(H)
     print("free %.1f GB, idle pool %.2f GB" % (fr / 1e9, idle / 1e9), flush=True)
     assert idle > 512 << 20
-    big = native.DeviceArray(H, fr + idle - (128 << 20), dtype=np.uint8)
-    big.free()
-    # the first context's idle blocks are now gone
-    assert mem_info(H)[1] < idle
-    # refill the pool, then allocate through a second context
+    # a second context on the device sees the first one's idle blocks
+    H2 = Hamiltonian(_models.gaas_box(cells=(2, 2, 2)), layout=None)
+    assert mem_info(H2)[1] == idle
+    with pytest.raises(native.NHError):
+        native.DeviceArray(H2, 4 * (fr + idle), dtype=np.uint8)
+    assert mem_info(H)[1] == 0  # the failed request took the idle blocks back
+    H2.close()
+    # refill the pool, then allocate across it
     refine_eigenpairs(H, X, np.linspace(0.5, 1.5, 8), depth=12, max_rounds=1)
     fr, idle = mem_info(H)
-    H2 = Hamiltonian(_models.gaas_box(cells=(2, 2, 2)), layout=None)
-    assert mem_info(H2)[1] == idle  # idle blocks of every context on the device
-    big = native.DeviceArray(H2, fr + idle - (128 << 20) - (64 << 20), dtype=np.uint8)
+    big = native.DeviceArray(H, fr + idle - (128 << 20), dtype=np.uint8)
     big.free()
-    H2.close()
     r = refine_eigenpairs(H, X[:2], np.linspace(0.5, 1.5, 2), depth=4, max_rounds=1)
     assert np.isfinite(r["residuals"]).all()
     H.close()
