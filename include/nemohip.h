@@ -167,6 +167,17 @@ nh_status nh_random_start(int kind, const int32_t *seed, int64_t n,
  * Lanczos inner products, lanczos.c:679-754). */
 nh_status nh_dot(nh_context *ctx, int64_t n, const nh_cplx *x,
                  const nh_cplx *y, nh_cplx *out /* host: sum conj(x) y */);
+/* nh_dot without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a fixed file-scope device array (2 x 2048 + 2
+ * doubles, 32 KB per device, held until the process ends) whose address
+ * is resolved once per device by nh_context_create; any n is accepted (at
+ * most 2048 partial sums are formed).  x, y: device pointers; out: host.
+ * Every call on one device shares that array, so calls that overlap on one
+ * device (other host threads, other contexts of the device) race. */
+nh_status nh_dot_noalloc(nh_context *ctx, int64_t n, const nh_cplx *x,
+                         const nh_cplx *y, nh_cplx *out);
 nh_status nh_nrm2(nh_context *ctx, int64_t n, const nh_cplx *x,
                   double *out /* host */);
 /* y = a x + b y (a, b complex) */

@@ -255,6 +255,7 @@ static nh_status create(const nh_model *m, int device, nh_context **out) {
   NH_CHECK(!cations.empty(), NH_ERR_ARG, "no cation-role atoms");
 
   NH_HIP(hipSetDevice(device));
+  NH_TRY(init_noalloc_scratch(device));
   nh_context *c = new nh_context();
   c->device = device;
   c->nb = m->nb;
@@ -456,6 +457,17 @@ nh_status nh_dot(nh_context *ctx, int64_t n, const nh_cplx *x,
   NH_CHECK(x && y && out && n >= 0, NH_ERR_ARG, "nh_dot: bad arguments");
   double r[2];
   NH_TRY(reduce_dot(ctx, n, x, y, r));
+  out->re = r[0];
+  out->im = r[1];
+  return NH_OK;
+}
+
+nh_status nh_dot_noalloc(nh_context *ctx, int64_t n, const nh_cplx *x,
+                         const nh_cplx *y, nh_cplx *out) {
+  NH_TRY(check_ctx(ctx));
+  NH_CHECK(x && y && out && n >= 0, NH_ERR_ARG, "nh_dot_noalloc: bad arguments");
+  double r[2];
+  NH_TRY(reduce_dot_noalloc(ctx, n, x, y, r));
   out->re = r[0];
   out->im = r[1];
   return NH_OK;

@@ -206,6 +206,11 @@ nh_status reduce_dot(nh_context *c, int64_t n, const nh_cplx *x,
                      const nh_cplx *y, double *out2 /* host re,im */);
 nh_status reduce_nrm2sq(nh_context *c, int64_t n, const nh_cplx *x,
                         double *out);
+// nh_dot_noalloc (nh_vector.hip): file-scope scratch per device, its address
+// resolved once per device at context creation
+nh_status init_noalloc_scratch(int device);
+nh_status reduce_dot_noalloc(nh_context *c, int64_t n, const nh_cplx *x,
+                             const nh_cplx *y, double *out2);
 nh_status launch_axpby(nh_context *c, int64_t n, nh_cplx a, const nh_cplx *x,
                        nh_cplx b, nh_cplx *y);
 nh_status ensure_reduce(nh_context *c, int blocks);

@@ -1,10 +1,41 @@
 // Vector kernels of the solvers: dot products and norms with FP64
 // accumulation (wavefront shuffles, then one partial per workgroup and a
 // single-workgroup final sum, so results are deterministic), and axpby.
+#include <mutex>
+
 #include "nh_internal.h"
 #include "nh_vector.h"
 
 namespace nh {
+
+// Scratch of nh_dot_noalloc: the partial sums of the dot product and its
+// result, in a file-scope device array (each device holds its own copy
+// until the process ends).  Host code uses the address that
+// hipGetSymbolAddress gives for the current device, never the array's name.
+constexpr int kNoAllocParts = kRedMaxBlocks;
+static __device__ __align__(256) double scratch[2 * kNoAllocParts + 2];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices] = {};
+
+// Resolves the device address of `scratch` on `device` (current device);
+// called once per device by nh_context_create.
+nh_status init_noalloc_scratch(int device) {
+  NH_CHECK(device >= 0 && device < kMaxDevices, NH_ERR_ARG,
+           "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch[device]) return NH_OK;
+  void *ptr = nullptr;
+  NH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  NH_CHECK(ptr, NH_ERR_HIP, "hipGetSymbolAddress(scratch) gave no address");
+  g_scratch[device] = static_cast<double *>(ptr);
+  return NH_OK;
+}
+
+double *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
 
 nh_status ensure_reduce(nh_context *c, int blocks) {
   if (blocks <= c->red_cap) return NH_OK;
@@ -87,6 +118,29 @@ nh_status reduce_dot(nh_context *c, int64_t n, const nh_cplx *x,
   NH_HIP(hipStreamSynchronize(c->stream));
   out2[0] = c->hred[0];
   out2[1] = c->hred[1];
+  return NH_OK;
+}
+
+// reduce_dot with the partial sums and the result in the device's file-scope
+// scratch (no allocation); the result is copied to the host out2.
+nh_status reduce_dot_noalloc(nh_context *c, int64_t n, const nh_cplx *x,
+                             const nh_cplx *y, double *out2) {
+  double *ws = noalloc_scratch(c->device);
+  NH_CHECK(ws, NH_ERR_STATE, "nh_dot_noalloc: scratch of device %d not set up",
+           c->device);
+  const int nb = reduce_blocks(n);
+  NH_CHECK(nb <= kNoAllocParts, NH_ERR_ARG,
+           "nh_dot_noalloc: %d partial sums exceed the scratch (%d)", nb,
+           kNoAllocParts);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kRedThreads), 0, c->stream,
+                     n, reinterpret_cast<const double2 *>(x),
+                     reinterpret_cast<const double2 *>(y), ws);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, c->stream,
+                     nb, ws, ws + 2 * (size_t)kNoAllocParts);
+  NH_HIP(hipGetLastError());
+  NH_HIP(hipMemcpyAsync(out2, ws + 2 * (size_t)kNoAllocParts, 2 * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
+  NH_HIP(hipStreamSynchronize(c->stream));
   return NH_OK;
 }
 

@@ -173,6 +173,7 @@ SIGNATURES = {
     "nh_memcpy_h2d": (ctypes.c_int, [P, P, P, ctypes.c_int64]),
     "nh_memcpy_d2h": (ctypes.c_int, [P, P, P, ctypes.c_int64]),
     "nh_dot": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
+    "nh_dot_noalloc": (ctypes.c_int, [P, ctypes.c_int64, P, P, P]),
     "nh_nrm2": (ctypes.c_int, [P, ctypes.c_int64, P, P]),
     "nh_axpby": (ctypes.c_int, [P, ctypes.c_int64, NHCplx, P, NHCplx, P]),
     "nh_eigensolve_lanczos": (ctypes.c_int, [P, P, P]),
