@@ -66,12 +66,13 @@ def build_pyramid4m():
 
 
 # hnorm: spectral radius of H (Lanczos Ritz extremes -12.4 / 39.4 eV and
-# -12.8 / 38.4 eV); windows: 4 VB / 4 CB levels at the gap edges (found by
-# a 3,000-step Lanczos scan of each structure)
+# -12.8 / 38.4 eV); windows: end past the gap edges (found by a 9,000-step
+# Lanczos scan of each relaxed structure: dot 0.7230 / 1.4986 eV, pyramid
+# 0.5561 / 1.4516 eV) and hold more than 4 levels on each side
 WORKLOADS = {
     "dot786k": dict(
         build=build_dot786k, hnorm=40.0,
-        windows=dict(erange_vb=(0.60, 0.70), erange_cb=(1.45, 1.52)),
+        windows=dict(erange_vb=(0.64, 0.75), erange_cb=(1.47, 1.54)),
         text="InAs dome dot 20x5 nm + 1 ML WL in 30x30x20 nm GaAs"),
     "pyramid4m": dict(
         build=build_pyramid4m, hnorm=40.0,

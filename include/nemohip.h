@@ -93,11 +93,19 @@ enum { NH_SET_HV_STAGED = 1 };
  * halved down to 8 until the ring takes at most half the free memory.
  * The eigenvectors agree to rounding whatever the length. */
 enum { NH_SET_RING_STEPS = 2 };
-/* NH_SET_BLOCK_HV: products of several vectors (block H·V) in the SK and
- * matrix-free layouts: -1 (default) by measurement, 0 one launch per
- * column, 1 all columns in one launch with the bond records read once,
- * 2 the matrix-core kernel (FP32 products, 8 columns). */
+/* NH_SET_BLOCK_HV: products of several vectors (block H·V): -1 (default)
+ * the matrix-core kernel for the row-owner block layout and 0 for the SK
+ * and matrix-free layouts (the fastest measured there), 0 one column per
+ * grid row, 1 all columns in one launch with the bond records read once
+ * (SK and matrix-free), 2 the matrix-core kernel (FP32 SK or block
+ * layouts, 8 columns). */
 enum { NH_SET_BLOCK_HV = 3 };
+/* NH_SET_SK_ROLES: workgroups of the SK and matrix-free products: 1 holds
+ * cation-role and anion-role atoms of the same cells together (both rows of
+ * a bond read its record in one workgroup), 0 one role per workgroup (the
+ * two roles of the same cells back to back on one XCD), -1 the default.
+ * The products are identical either way. */
+enum { NH_SET_SK_ROLES = 4 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 
 /* Assemble H on the device in the given layout and precision.  Replaces
@@ -378,6 +386,33 @@ nh_status nh_vff_energy_grad(nh_context *ctx, const double *alpha,
 nh_status nh_vff_relax(nh_context *ctx, const double *alpha,
                        const double *beta, const double *a0, int max_iter,
                        double gtol, double *disp_out, double *info);
+
+/* The strain models and boundaries of the reference's strain block
+ * (Strain_struct: StrainModel, Keating):
+ *   NH_VFF_KEATING          alpha, beta constant (VFF_keating);
+ *   NH_VFF_KEATING_STRAINED alpha and beta scaled with the bond's strain,
+ *     StrainParam::alpha_eff / beta_eff (StrainParam.c:60-214) with the
+ *     acoustic anharmonicity constants aa, ba, ca (the material's
+ *     strain_VFF[0..2], StrainParam.c:265-280); the effective constants
+ *     are held fixed in the gradient, as calc_estrain does
+ *     (strain3d.c:1687-1700, 1835-1860);
+ *   fix_surface 1: atoms with fewer than four neighbours stay fixed
+ *     (FixSurfaceAll); 0: every atom moves (AllFree).
+ * aa, ba, ca may be NULL for NH_VFF_KEATING. */
+enum { NH_VFF_KEATING = 0, NH_VFF_KEATING_STRAINED = 1 };
+typedef struct nh_vff_params {
+  const double *alpha, *beta, *a0; /* [S][S] */
+  const double *aa, *ba, *ca;      /* [S][S] */
+  int32_t model;
+  int32_t fix_surface;
+} nh_vff_params;
+/* nh_vff_energy_grad / nh_vff_relax with the model and boundary given. */
+nh_status nh_vff_energy_grad_model(nh_context *ctx, const nh_vff_params *p,
+                                   const double *disp, double *grad_out,
+                                   double *energy);
+nh_status nh_vff_relax_model(nh_context *ctx, const nh_vff_params *p,
+                             int max_iter, double gtol, double *disp_out,
+                             double *info);
 
 /* ---- learned warm start: 3-D CNN on the cation grid ----------------------
  * L conv layers 3x3x3 (width 32, ReLU) + 1x1x1 head; FP32 master weights,

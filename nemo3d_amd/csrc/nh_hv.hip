@@ -7,19 +7,20 @@ namespace nh {
 namespace {
 
 // SK-compressed and matrix-free layouts (see nh_hv_sk.h; SRC = SkSrc or
-// MfSrc); workgroup -> cation or anion block by sk_block_of.
+// MfSrc); workgroup -> atoms by sk_lane_of.
 template <int NSP, typename R, bool QUAD, typename SRC>
 __global__ __launch_bounds__(kSkThreads) void k_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
-    const int32_t *__restrict__ anions, int ppx,
+    const int32_t *__restrict__ anions, SkGrid grid,
     const double *__restrict__ onsite, const SRC src,
     const int32_t *__restrict__ nbr, double shift,
     const double2 *__restrict__ x, double2 *__restrict__ y, int64_t ld) {
   constexpr int NB = NSP * 10;
-  const SkBlock wb = sk_block_of(blockIdx.x, ppx);
-  const bool cat = wb.cat;
-  const int bl = wb.bl;
-  const int li = bl * kSkAtomsPerBlock + threadIdx.x / 4;
+  const SkLane ln = sk_lane_of(grid, blockIdx.x, threadIdx.x);
+  // the role is uniform across the wavefront; read as a scalar, the role
+  // branches below are scalar branches (one per wave)
+  const bool cat = __builtin_amdgcn_readfirstlane((int)ln.cat) != 0;
+  const int li = ln.li;
   const int slot = threadIdx.x & 3;
   const bool active = li < (cat ? nc : na);
   const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
@@ -127,13 +128,17 @@ __device__ __forceinline__ void sk_columns(int atom, int slot, bool active,
 template <int NSP, typename R, typename SRC>
 __global__ __launch_bounds__(kSkThreads) void k_hV_sk(
     int nc, const int32_t *__restrict__ cations, int na,
-    const int32_t *__restrict__ anions, int ppx,
+    const int32_t *__restrict__ anions, SkGrid grid,
     const double *__restrict__ onsite, const SRC src,
     const int32_t *__restrict__ nbr, double shift,
     const double2 *__restrict__ X, double2 *__restrict__ Y, int64_t ld, int ncol) {
-  const SkBlock wb = sk_block_of(blockIdx.x, ppx);
-  const bool cat = wb.cat;
-  const int li = wb.bl * kSkAtomsPerBlock + threadIdx.x / 4;
+  // one role per workgroup (grid.mixed == 0): the role branch below holds
+  // the column loop's barriers
+  const SkLane ln = sk_lane_of(grid, blockIdx.x, threadIdx.x);
+  // the role is uniform across the wavefront; read as a scalar, the role
+  // branches below are scalar branches (one per wave)
+  const bool cat = __builtin_amdgcn_readfirstlane((int)ln.cat) != 0;
+  const int li = ln.li;
   const int slot = threadIdx.x & 3;
   const bool active = li < (cat ? nc : na);
   const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
@@ -185,19 +190,26 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
            "H·v: Hamiltonian not built in a supported layout");
   NH_CHECK(ncol >= 1 && ncol <= 65535 && ld >= c->n, NH_ERR_ARG,
            "H·V: bad ncol %d / ld %lld", ncol, (long long)ld);
-  // block products: the matrix-core kernel (FP32, 8 columns) or all the
-  // columns in one SK / matrix-free launch
+  // block products: the matrix-core kernel (FP32, 8 columns), all the
+  // columns in one SK / matrix-free launch with the bond records read once,
+  // or one column per grid row.  Default (-1): the matrix-core kernel for
+  // the row-owner block layout; one column per grid row for the SK and
+  // matrix-free layouts, measured fastest on a 786,520-atom dot (8 columns:
+  // 2.46 ms FP64 / 2.12 ms FP32 against 4.38 / 2.53 ms in one launch and
+  // 4.02 ms on the matrix cores)
   const int mode = c->block_hv;
-  if ((mode == 2 || mode == -1) && hV_mfma_supported(c, ncol))
+  if ((mode == 2 || (mode == -1 && c->layout == NH_LAYOUT_BLOCKS)) &&
+      hV_mfma_supported(c, ncol))
     return launch_hV_mfma(c, shift, x, y, ld);
-  if (sk_like(c) && ncol > 1 && mode != 0) {
-    const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
+  if (sk_like(c) && ncol > 1 && mode == 1) {
+    int blocks = 0;
+    const SkGrid sg = sk_grid(c->n_cations, c->n_anions, false, &blocks);
     const double2 *xd = reinterpret_cast<const double2 *>(x);
     double2 *yd = reinterpret_cast<double2 *>(y);
 #define HVK(NSP, R, SRC)                                                      \
-  hipLaunchKernelGGL((k_hV_sk<NSP, R, decltype(SRC)>), dim3(16 * ppx),        \
+  hipLaunchKernelGGL((k_hV_sk<NSP, R, decltype(SRC)>), dim3(blocks),          \
                      dim3(kSkThreads), 0, c->stream, c->n_cations,            \
-                     c->cations, c->n_anions, c->anions, ppx, c->onsite, SRC, \
+                     c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,  \
                      c->nbr, shift, xd, yd, ld, ncol)
 #define HVP(R, SRC)                                                          \
   do {                                                                       \
@@ -218,14 +230,15 @@ nh_status launch_hv(nh_context *c, double shift, const nh_cplx *x, nh_cplx *y,
     return NH_OK;
   }
   if (sk_like(c)) {
-    const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
-    const dim3 g(16 * ppx, ncol);
+    int blocks = 0;
+    const SkGrid sg = sk_grid(c->n_cations, c->n_anions, sk_mixed_roles(c), &blocks);
+    const dim3 g(blocks, ncol);
     const double2 *xd = reinterpret_cast<const double2 *>(x);
     double2 *yd = reinterpret_cast<double2 *>(y);
 #define SKL1(NSP, R, SRC, QUAD)                                               \
   hipLaunchKernelGGL((k_hv_sk<NSP, R, QUAD, decltype(SRC)>), g,               \
                      dim3(kSkThreads), 0, c->stream, c->n_cations,            \
-                     c->cations, c->n_anions, c->anions, ppx, c->onsite, SRC, \
+                     c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,  \
                      c->nbr, shift, xd, yd, ld)
 #define SKL(NSP, R, SRC)                                                     \
   do {                                                                       \

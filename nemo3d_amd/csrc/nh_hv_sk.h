@@ -24,26 +24,45 @@ namespace nh {
 constexpr int kSkThreads = 256;
 constexpr int kSkAtomsPerBlock = kSkThreads / 4;
 
-// Workgroup -> (role, block of the cation or anion list).  Cation block k
-// and anion block k cover the same cells (both lists are in atom order), so
-// they run back to back on the same XCD and share the neighbours' vector
-// entries in its L2: consecutive workgroups go to different XCDs (round
-// robin over 8), workgroup b runs on XCD b % 8, and XCD x takes the blocks
-// [x * ppx, (x + 1) * ppx), alternating cation and anion.  Grid: 16 * ppx,
-// ppx = ceil(max(cation blocks, anion blocks) / 8).
-struct SkBlock {
-  bool cat;
-  int bl;
+// Workgroup -> atoms.  Consecutive workgroups go to different XCDs (round
+// robin over 8): workgroup b runs on XCD b % 8, and XCD x takes the blocks
+// [x * ppx, (x + 1) * ppx) of the atom lists, so each XCD works on a
+// compact region (the lists are ordered by bricks of cells) and reuses the
+// neighbours' vector entries from its L2.  Cation block k and anion block k
+// cover the same cells.  Two mappings:
+//  * one role per workgroup (mixed = 0): 64 atoms, cation and anion blocks
+//    alternating on each XCD, so the two roles of the same cells run back
+//    to back; grid 16 * ppx, ppx = ceil(max(blocks of 64) / 8);
+//  * both roles (mixed = 1): waves 0-1 hold 32 cations and waves 2-3 the 32
+//    anions of the same list position, so both rows of a bond read its
+//    record within one workgroup; grid 8 * ppx, ppx = ceil(max(blocks of
+//    32) / 8).
+// Either way the role is uniform across a wavefront, so whether a lane
+// applies T (anion row) or its transpose (cation row) is too.
+struct SkGrid {
+  int ppx;
+  int mixed;
 };
-__device__ __forceinline__ SkBlock sk_block_of(int b, int ppx) {
+struct SkLane {
+  bool cat;
+  int li;  // index into the cation or anion list
+};
+__device__ __forceinline__ SkLane sk_lane_of(SkGrid g, int b, int t) {
   const int x = b & 7, s = b >> 3;
-  return {(s & 1) == 0, x * ppx + (s >> 1)};
+  if (g.mixed) {
+    constexpr int kHalf = kSkThreads / 2;
+    return {t < kHalf, (x * g.ppx + s) * (kSkAtomsPerBlock / 2) + (t & (kHalf - 1)) / 4};
+  }
+  return {(s & 1) == 0, (x * g.ppx + (s >> 1)) * kSkAtomsPerBlock + t / 4};
 }
-inline int sk_pairs_per_xcd(int n_cations, int n_anions) {
-  const int nbc = (n_cations + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
-  const int nba = (n_anions + kSkAtomsPerBlock - 1) / kSkAtomsPerBlock;
-  const int nbmax = nbc > nba ? nbc : nba;
-  return (nbmax + 7) / 8;
+// the grid of a launch and its mapping
+inline SkGrid sk_grid(int n_cations, int n_anions, bool mixed, int *blocks) {
+  const int per = mixed ? kSkAtomsPerBlock / 2 : kSkAtomsPerBlock;
+  const int nbc = (n_cations + per - 1) / per;
+  const int nba = (n_anions + per - 1) / per;
+  const int ppx = ((nbc > nba ? nbc : nba) + 7) / 8;
+  *blocks = (mixed ? 8 : 16) * ppx;
+  return SkGrid{ppx, mixed ? 1 : 0};
 }
 
 // Onsite contribution (diagonal - shift, surface s-p couplings, spin-orbit)
@@ -424,7 +443,9 @@ constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 // XT: double2 (the FP64 vector) or float2 (its FP32 copy, R = float).
 // QUAD: gather through LDS (sk_quad_gather; every thread of the workgroup
 // must call, there is a barrier) in the caller's LDS area L, or directly
-// from the caches.
+// from the caches.  Callers branch on the role with a wave-uniform
+// (scalar) condition, so every wave runs exactly one instantiation and
+// reaches the same number of barriers whichever role it holds.
 template <int NSP, typename R, bool CATION, typename XT, bool QUAD, typename SRC>
 __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
                                          const SRC &src,

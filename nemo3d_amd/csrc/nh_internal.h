@@ -88,6 +88,7 @@ struct nh_context {
   int hv_staged = -1;  // NH_SET_HV_STAGED (-1: by layout, see use_staged)
   int ring_steps = 0;  // NH_SET_RING_STEPS (0: by free memory)
   int block_hv = -1;   // NH_SET_BLOCK_HV (-1: by measurement)
+  int sk_roles = -1;   // NH_SET_SK_ROLES (-1: default, sk_mixed_roles)
 
   // assembled Hamiltonian
   int layout = -1;
@@ -134,6 +135,13 @@ inline bool sk_like(const nh_context *c) {
 }
 inline bool use_staged(const nh_context *c) {
   return c->hv_staged < 0 ? sk_like(c) : c->hv_staged != 0;
+}
+// SK / matrix-free products: both roles in one workgroup (waves 0-1 the
+// cations, waves 2-3 the anions at the same list position, so both rows of
+// a bond read its record within one workgroup) or one role per workgroup
+constexpr bool kSkMixedRolesDefault = false;
+inline bool sk_mixed_roles(const nh_context *c) {
+  return c->sk_roles < 0 ? kSkMixedRolesDefault : c->sk_roles != 0;
 }
 }  // namespace nh
 

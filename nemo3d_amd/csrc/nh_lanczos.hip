@@ -143,7 +143,7 @@ __device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
 template <int NSP, typename R, bool QUAD, bool FUSE, typename SRC>
 __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
-    const int32_t *__restrict__ anions, int ppx,
+    const int32_t *__restrict__ anions, SkGrid grid,
     const double *__restrict__ onsite, const SRC src,
     const int32_t *__restrict__ nbr, const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
     double2 *__restrict__ Q, double2 *__restrict__ Y,
@@ -152,10 +152,11 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
     const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
   constexpr int NB = NSP * 10;
   constexpr int NO = (NB + 3) / 4;
-  const SkBlock wb = sk_block_of(blockIdx.x, ppx);
-  const bool cat = wb.cat;
-  const int bl = wb.bl;
-  const int li = bl * kSkAtomsPerBlock + threadIdx.x / 4;
+  const SkLane ln = sk_lane_of(grid, blockIdx.x, threadIdx.x);
+  // the role is uniform across the wavefront; read as a scalar, the role
+  // branches below are scalar branches (one per wave)
+  const bool cat = __builtin_amdgcn_readfirstlane((int)ln.cat) != 0;
+  const int li = ln.li;
   const int slot = threadIdx.x & 3;
   const bool active = li < (cat ? nc : na);
   const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
@@ -687,9 +688,10 @@ nh_status LanczosRun::upload_start(float2 *f32) {
 
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
-  const int ppx = sk_pairs_per_xcd(c->n_cations, c->n_anions);
+  int sk_blocks = 0;
+  const SkGrid sg = sk_grid(c->n_cations, c->n_anions, sk_mixed_roles(c), &sk_blocks);
   const int gridA = sk_like(c)
-                        ? 16 * ppx
+                        ? sk_blocks
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
   // q_k is stored (ring slot) only in the eigenvector pass, which
   // accumulates from it; the update forms q_k, q_{k-1} from r_k, r_{k-1}
@@ -734,7 +736,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #define LZ_SK2(NSP, R, SRC, QUAD, FUSE)                                         \
   hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE, decltype(SRC)>),           \
                      dim3(gridA), dim3(kSkThreads), 0, c->stream, c->n_cations, \
-                     c->cations, c->n_anions, c->anions, ppx, c->onsite, SRC,   \
+                     c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,    \
                      c->nbr, Rk, g32, Q, Yk, bprev, part, alpha_ + k,           \
                      k > 0 ? Rkm1 : nullptr, bprev2, y32)
 #define LZ_SK1(NSP, R, SRC, QUAD)                                               \

@@ -9,12 +9,31 @@
 // of the pair (StrainParam).  The total is sum_i E_i, so each bond term
 // enters from both of its atoms.
 //
+// The gradient is the reference's (strain3d.c:1806-1830 and 1889-2050): an
+// atom receives the derivative of its own copy of each bond term only
+// (-4 a/(2 d0^2) dl r_in), half the derivative of E, and the full
+// derivative of the angle terms.  The relaxation follows that gradient, so
+// its stationary point is that of E with each bond term counted once; E
+// itself is reported as the reference prints it.
+//
+// StrainModel = VFF_keating_strained replaces a and sqrt(b_in b_ip) by the
+// strain-scaled constants of StrainParam::alpha_eff / beta_eff
+// (StrainParam.c:60-214): per bond a (1 - A (|r|^2 / d0^2 - 1)), per angle
+// b_in (1 - B (cos + 1/3)) (1 - C (|r_in| |r_ip| / (|d0_in| |d0_ip|) - 1))
+// with B, C the geometric means of the two bonds' constants and both
+// factors truncated near their extrema; the reference's b_in is the
+// constant of the lower-slot leg n.  As in calc_estrain (strain3d.c:
+// 1687-1700, 1835-1860) the scaled constants are taken as fixed when the
+// gradient is formed, so the "gradient" of this model is that of the
+// harmonic form at the current constants.
+//
 // The gradient on atom i is gathered per atom (no atomics): its own bond
 // and angle terms plus the angle terms centred on each neighbour j in which
-// i is a leg.  Atoms with fewer than four neighbours are held fixed
-// (Keating = FixSurfaceAll).  Minimisation: Polak-Ribiere+ nonlinear
-// conjugate gradients with a secant line search on the directional
-// derivative (the reference uses MacKay's macopt CG, eigen/src/macopt.c).
+// i is a leg.  Keating = FixSurfaceAll holds atoms with fewer than four
+// neighbours fixed; AllFree moves every atom.  Minimisation: Polak-Ribiere+
+// nonlinear conjugate gradients with a secant line search on the
+// directional derivative (the reference uses MacKay's macopt CG,
+// eigen/src/macopt.c, whose line search also uses only gradients).
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -29,7 +48,67 @@ struct VffArgs {
   const int32_t *species, *nbr, *bond_q;
   const double *alpha, *beta, *a0;  // [S][S]
   double lx, ly, lz;
+  const double *aa, *ba, *ca;  // [S][S], NH_VFF_KEATING_STRAINED only
+  int model;
+  int fix_surface;
 };
+
+// StrainParam::alpha_eff with anh = 1 (StrainParam.c:60-100); the
+// reference's how_close_to_maximum_of_distorsion_energy is 0.001
+constexpr double kVffNearMax = 0.001;
+
+__device__ __forceinline__ double alpha_eff(double al0, double A, double r2,
+                                            double d2) {
+  const double delt = 1.0 - kVffNearMax;
+  if (r2 < d2 * (1.0 + delt * 2.0 / (3.0 * A)))
+    return al0 * (1.0 - A * (r2 / d2 - 1.0));
+  return al0 * (1.0 - delt / 1.5);
+}
+
+// StrainParam::beta_eff with anh = 1 (StrainParam.c:118-214): be0 the
+// constant of the first leg, d1/d2 the ideal and r1/r2 the strained legs
+__device__ __forceinline__ double beta_eff(double be0, double B, double C,
+                                           const double d1[3],
+                                           const double d2[3],
+                                           const double r1[3],
+                                           const double r2[3]) {
+  const double d10 = sqrt(d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2]);
+  const double d20 = sqrt(d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2]);
+  const double rr = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  const double r10 = sqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+  const double r20 = sqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+  const double rd = r10 * r20 / (d10 * d20);
+  const double tt = rr / (r10 * r20);
+  const double delt = 1.0 + kVffNearMax, delh = 1.0 - kVffNearMax;
+  double beta = be0;
+  bool flag = true;
+  if (tt > delt / 3.0 * (2.0 * (1.0 / B - 1.0 / 3.0) - 1.0 / (3.0 * rd))) {
+    beta = be0 * ((1.0 - 2.0 * delt / 3.0) +
+                  B / 3.0 * (delt / 3.0 * (1.0 / rd + 2.0) - 1.0)) *
+           (1.0 - C * (rd - 1.0));
+    flag = false;
+  }
+  if (rd > 1.0 + delh * 2.0 / (3.0 * C)) {
+    const double be = be0 * (1.0 - B * (tt + 1.0 / 3.0)) * (1.0 - delh * 2.0 / 3.0);
+    if (flag || be > beta) beta = be;
+    flag = false;
+  }
+  if (flag) beta = be0 * (1.0 - B * (tt + 1.0 / 3.0)) * (1.0 - C * (rd - 1.0));
+  return beta;
+}
+
+// angle constant of the legs n1 < n2 (slots) centred on an atom of type tc
+__device__ __forceinline__ double angle_const(const VffArgs &g, int tc, int t1,
+                                              int t2, const double d1[3],
+                                              const double d2[3],
+                                              const double r1[3],
+                                              const double r2[3]) {
+  const double b1 = g.beta[tc * g.S + t1], b2 = g.beta[tc * g.S + t2];
+  if (g.model == 0) return sqrt(b1 * b2);
+  const double B = sqrt(g.ba[tc * g.S + t1] * g.ba[tc * g.S + t2]);
+  const double C = sqrt(g.ca[tc * g.S + t1] * g.ca[tc * g.S + t2]);
+  return beta_eff(sqrt(b1 * b1), B, C, d1, d2, r1, r2);
+}
 
 __device__ __forceinline__ void bond_vec(const VffArgs &g, const double *u,
                                          int i, int n, int j, double r[3],
@@ -70,18 +149,24 @@ __global__ __launch_bounds__(kRedThreads) void k_vff_grad(
     // terms centred on i
     for (int n = 0; n < 4; ++n) {
       if (nb[n] < 0) continue;
-      const double al = g.alpha[ti * g.S + g.species[nb[n]]];
+      const int tn = g.species[nb[n]];
       const double d0sq = dot3(d0[n], d0[n]);
-      const double dl = dot3(r[n], r[n]) - d0sq;
+      const double rsq = dot3(r[n], r[n]);
+      const double al =
+          g.model == 0 ? g.alpha[ti * g.S + tn]
+                       : alpha_eff(g.alpha[ti * g.S + tn], g.aa[ti * g.S + tn],
+                                   rsq, d0sq);
+      const double dl = rsq - d0sq;
       e_i += 0.5 * al / d0sq * dl * dl;
-      // this term and the same bond's term on the neighbour (equal value):
-      // d/du_i of 2 * (a/2d0^2) dl^2 = -(4 a/d0^2) dl r_in
-      const double f = 4.0 * al / d0sq * dl;
+      // the derivative of this atom's copy of the bond term: d/du_i of
+      // (a/2d0^2) dl^2 = -(2 a/d0^2) dl r_in (the neighbour's copy is
+      // not differentiated here, as in the reference)
+      const double f = 2.0 * al / d0sq * dl;
       for (int k = 0; k < 3; ++k) gr[k] -= f * r[n][k];
       for (int p = n + 1; p < 4; ++p) {
         if (nb[p] < 0) continue;
-        const double be = sqrt(g.beta[ti * g.S + g.species[nb[n]]] *
-                               g.beta[ti * g.S + g.species[nb[p]]]);
+        const double be = angle_const(g, ti, tn, g.species[nb[p]], d0[n],
+                                      d0[p], r[n], r[p]);
         const double K = be / sqrt(dot3(d0[p], d0[p]) * d0sq);
         const double t = dot3(r[n], r[p]) - dot3(d0[n], d0[p]);
         e_i += K * t * t;
@@ -99,19 +184,22 @@ __global__ __launch_bounds__(kRedThreads) void k_vff_grad(
       for (int q = 0; q < 4; ++q)
         if (g.nbr[j * 4 + q] == i) si = q;
       bond_vec(g, u, j, si, i, rji, d0ji);
-      const double b_ji = g.beta[tj * g.S + ti];
       for (int q = 0; q < 4; ++q) {
         const int k2 = g.nbr[j * 4 + q];
         if (q == si || k2 < 0) continue;
         double rjk[3], d0jk[3];
         bond_vec(g, u, j, q, k2, rjk, d0jk);
-        const double be = sqrt(b_ji * g.beta[tj * g.S + g.species[k2]]);
+        const int tk = g.species[k2];
+        // the angle's legs in j's slot order
+        const double be =
+            si < q ? angle_const(g, tj, ti, tk, d0ji, d0jk, rji, rjk)
+                   : angle_const(g, tj, tk, ti, d0jk, d0ji, rjk, rji);
         const double K = be / sqrt(dot3(d0ji, d0ji) * dot3(d0jk, d0jk));
         const double t = dot3(rji, rjk) - dot3(d0ji, d0jk);
         for (int k = 0; k < 3; ++k) gr[k] += 2.0 * K * t * rjk[k];
       }
     }
-    const bool fixed = present < 4;
+    const bool fixed = g.fix_surface && present < 4;
     for (int k = 0; k < 3; ++k) grad[i * 3 + k] = fixed ? 0.0 : gr[k];
   }
   if (part) {
@@ -283,68 +371,124 @@ nh_status Vff::run(int max_iter, double gtol, double *out_disp, double *info) {
 }  // namespace
 }  // namespace nh
 
+namespace nh {
+namespace {
+
+// The parameter tables of a model on the device (six [S][S] tables in one
+// allocation), freed with the object.
+class VffTables {
+ public:
+  ~VffTables() { hipFree(mem_); }
+  nh_status upload(nh_context *c, const nh_vff_params *p) {
+    NH_CHECK(p && p->alpha && p->beta && p->a0 &&
+                 (p->model == NH_VFF_KEATING ||
+                  (p->model == NH_VFF_KEATING_STRAINED && p->aa && p->ba && p->ca)) &&
+                 (p->fix_surface == 0 || p->fix_surface == 1),
+             NH_ERR_ARG, "VFF: bad parameters (model %d, fix_surface %d)",
+             p ? p->model : -1, p ? p->fix_surface : -1);
+    const size_t ss = (size_t)c->n_species * c->n_species;
+    NH_TRY(device_malloc(c, (void **)&mem_, 6 * ss * sizeof(double)));
+    const double *src[6] = {p->alpha, p->beta, p->a0, p->aa, p->ba, p->ca};
+    for (int k = 0; k < 6; ++k)
+      if (src[k])
+        NH_HIP(hipMemcpy(mem_ + k * ss, src[k], ss * sizeof(double),
+                         hipMemcpyHostToDevice));
+    args = VffArgs{c->n_atoms, c->n_species, c->species, c->nbr, c->bond_q,
+                   mem_, mem_ + ss, mem_ + 2 * ss, c->lattice[0], c->lattice[1],
+                   c->lattice[2], mem_ + 3 * ss, mem_ + 4 * ss, mem_ + 5 * ss,
+                   p->model, p->fix_surface};
+    return NH_OK;
+  }
+  VffArgs args{};
+
+ private:
+  double *mem_ = nullptr;
+};
+
+nh_status energy_grad(nh_context *ctx, const nh_vff_params *p,
+                      const double *disp, double *grad_out, double *energy) {
+  NH_CHECK(ctx && disp && grad_out && energy, NH_ERR_ARG,
+           "nh_vff_energy_grad: bad arguments");
+  (void)hipGetLastError();
+  NH_HIP(hipSetDevice(ctx->device));
+  VffTables t;
+  NH_TRY(t.upload(ctx, p));
+  const int A = ctx->n_atoms;
+  const int blocks = (A + kRedThreads - 1) / kRedThreads;
+  double *m[3] = {nullptr, nullptr, nullptr};
+  const size_t sz[3] = {(size_t)A * 3, (size_t)A * 3, (size_t)(blocks + 2) * 2 + 8};
+  nh_status st = NH_OK;
+  for (int k = 0; k < 3 && st == NH_OK; ++k)
+    st = device_malloc(ctx, (void **)&m[k], sz[k] * sizeof(double));
+  hipError_t e = hipSuccess;
+  double res[2] = {0, 0};
+  if (st == NH_OK) {
+    e = hipMemcpy(m[0], disp, sz[0] * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_vff_grad, dim3(blocks), dim3(kRedThreads), 0,
+                         ctx->stream, t.args, m[0], m[1], m[2]);
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0,
+                         ctx->stream, blocks, m[2], m[2] + 2 * (blocks + 1));
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpy(grad_out, m[1], sz[1] * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(res, m[2] + 2 * (blocks + 1), 2 * sizeof(double),
+                    hipMemcpyDeviceToHost);
+  }
+  for (double *q : m) hipFree(q);
+  NH_TRY(st);
+  NH_CHECK(e == hipSuccess, NH_ERR_HIP, "nh_vff_energy_grad: %s", hipGetErrorString(e));
+  *energy = res[0];
+  return NH_OK;
+}
+
+nh_status relax(nh_context *ctx, const nh_vff_params *p, int max_iter,
+                double gtol, double *disp_out, double *info) {
+  NH_CHECK(ctx && info && max_iter >= 0 && gtol >= 0, NH_ERR_ARG,
+           "nh_vff_relax: bad arguments");
+  (void)hipGetLastError();
+  NH_HIP(hipSetDevice(ctx->device));
+  VffTables t;
+  NH_TRY(t.upload(ctx, p));
+  Vff v(ctx, t.args);
+  return v.run(max_iter, gtol, disp_out, info);
+}
+
+nh_vff_params keating(const double *alpha, const double *beta, const double *a0) {
+  return nh_vff_params{alpha, beta, a0, nullptr, nullptr, nullptr,
+                       NH_VFF_KEATING, 1};
+}
+
+}  // namespace
+}  // namespace nh
+
 extern "C" nh_status nh_vff_energy_grad(nh_context *ctx, const double *alpha,
                                         const double *beta, const double *a0,
                                         const double *disp, double *grad_out,
                                         double *energy) {
-  NH_CHECK(ctx && alpha && beta && a0 && disp && grad_out && energy,
-           NH_ERR_ARG, "nh_vff_energy_grad: bad arguments");
-  (void)hipGetLastError();
-  NH_HIP(hipSetDevice(ctx->device));
-  const int S = ctx->n_species, A = ctx->n_atoms;
-  NH_TRY(nh::ensure_reduce(ctx, 1));
-  std::vector<double *> m(6, nullptr);
-  const size_t sz[6] = {(size_t)S * S, (size_t)S * S, (size_t)S * S,
-                        (size_t)A * 3, (size_t)A * 3,
-                        (size_t)(A / nh::kRedThreads + 2) * 2 + 8};
-  for (int k = 0; k < 6; ++k) NH_TRY(nh::device_malloc(ctx, (void **)&m[k], sz[k] * sizeof(double)));
-  hipMemcpy(m[0], alpha, sz[0] * sizeof(double), hipMemcpyHostToDevice);
-  hipMemcpy(m[1], beta, sz[1] * sizeof(double), hipMemcpyHostToDevice);
-  hipMemcpy(m[2], a0, sz[2] * sizeof(double), hipMemcpyHostToDevice);
-  hipMemcpy(m[3], disp, sz[3] * sizeof(double), hipMemcpyHostToDevice);
-  nh::VffArgs a{A, S, ctx->species, ctx->nbr, ctx->bond_q, m[0], m[1], m[2],
-                ctx->lattice[0], ctx->lattice[1], ctx->lattice[2]};
-  const int blocks = (A + nh::kRedThreads - 1) / nh::kRedThreads;
-  hipLaunchKernelGGL(nh::k_vff_grad, dim3(blocks), dim3(nh::kRedThreads), 0,
-                     ctx->stream, a, m[3], m[4], m[5]);
-  hipLaunchKernelGGL(nh::k_sum_partials, dim3(1), dim3(nh::kRedThreads), 0,
-                     ctx->stream, blocks, m[5], m[5] + 2 * (blocks + 1));
-  hipError_t e = hipGetLastError();
-  double res[2] = {0, 0};
-  if (e == hipSuccess) e = hipMemcpy(grad_out, m[4], sz[4] * sizeof(double), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(res, m[5] + 2 * (blocks + 1), 2 * sizeof(double), hipMemcpyDeviceToHost);
-  for (double *p : m) hipFree(p);
-  NH_CHECK(e == hipSuccess, NH_ERR_HIP, "nh_vff_energy_grad: %s", hipGetErrorString(e));
-  *energy = res[0];
-  return NH_OK;
+  const nh_vff_params p = nh::keating(alpha, beta, a0);
+  return nh::energy_grad(ctx, &p, disp, grad_out, energy);
 }
 
 extern "C" nh_status nh_vff_relax(nh_context *ctx, const double *alpha,
                                   const double *beta, const double *a0,
                                   int max_iter, double gtol, double *disp_out,
                                   double *info) {
-  NH_CHECK(ctx && alpha && beta && a0 && info && max_iter >= 0 && gtol >= 0,
-           NH_ERR_ARG, "nh_vff_relax: bad arguments");
-  (void)hipGetLastError();
-  NH_HIP(hipSetDevice(ctx->device));
-  const int S = ctx->n_species;
-  double *dal = nullptr, *dbe = nullptr, *da0 = nullptr;
-  NH_TRY(nh::device_malloc(ctx, (void **)&dal, S * S * sizeof(double)));
-  NH_TRY(nh::device_malloc(ctx, (void **)&dbe, S * S * sizeof(double)));
-  NH_TRY(nh::device_malloc(ctx, (void **)&da0, S * S * sizeof(double)));
-  hipMemcpy(dal, alpha, S * S * sizeof(double), hipMemcpyHostToDevice);
-  hipMemcpy(dbe, beta, S * S * sizeof(double), hipMemcpyHostToDevice);
-  hipMemcpy(da0, a0, S * S * sizeof(double), hipMemcpyHostToDevice);
-  nh::VffArgs a{ctx->n_atoms, S, ctx->species, ctx->nbr, ctx->bond_q,
-                dal, dbe, da0, ctx->lattice[0], ctx->lattice[1],
-                ctx->lattice[2]};
-  nh_status st;
-  {
-    nh::Vff v(ctx, a);
-    st = v.run(max_iter, gtol, disp_out, info);
-  }
-  hipFree(dal);
-  hipFree(dbe);
-  hipFree(da0);
-  return st;
+  const nh_vff_params p = nh::keating(alpha, beta, a0);
+  return nh::relax(ctx, &p, max_iter, gtol, disp_out, info);
+}
+
+extern "C" nh_status nh_vff_energy_grad_model(nh_context *ctx,
+                                              const nh_vff_params *p,
+                                              const double *disp,
+                                              double *grad_out, double *energy) {
+  return nh::energy_grad(ctx, p, disp, grad_out, energy);
+}
+
+extern "C" nh_status nh_vff_relax_model(nh_context *ctx, const nh_vff_params *p,
+                                        int max_iter, double gtol,
+                                        double *disp_out, double *info) {
+  return nh::relax(ctx, p, max_iter, gtol, disp_out, info);
 }

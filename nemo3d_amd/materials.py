@@ -47,20 +47,21 @@ def database():
 
 
 def find_material(name, band_model, source=None, spinless_view=False,
-                  tkel=300.0):
+                  tkel=300.0, records=None):
     """First matching record in deck order (i_mat.c:184-224).
 
     ``spinless_view`` lets a ``Bands_10_sp3d5ss_nospin`` request use a
     ``Bands_20_sp3d5ss_spin`` record (same 35 sp3d5s* parameters; the
     spin-orbit entries are unused without spin) when the decks hold no
-    spinless record for the material.
+    spinless record for the material.  ``records``: search these records
+    (a deck's own material list) instead of the shipped database.
     """
     models = [band_model]
     if spinless_view and band_model == BANDS_10_NOSPIN:
         models.append(BANDS_20_SPIN)
     for bm in models:
-        for rec in database():
-            if source is not None and rec["source"] != source:
+        for rec in (database() if records is None else records):
+            if source is not None and rec.get("source") != source:
                 continue
             if rec["material"] != name or rec["band_model"] != bm:
                 continue
@@ -96,7 +97,7 @@ class MaterialTables:
 
 
 def param_init(compounds, band_model, source=None, spinless_view=True,
-               include_strain=True):
+               include_strain=True, deck_records=None):
     """Build parmat/strnmat for a list of compound names (e.g. ['GaAs',
     'InAs']).  Species are indexed in first-seen order."""
     if not include_strain:
@@ -117,7 +118,7 @@ def param_init(compounds, band_model, source=None, spinless_view=True,
     records = {}
     for comp, c_el, a_el in pairs:
         rec = find_material(comp, band_model, source=source,
-                            spinless_view=spinless_view)
+                            spinless_view=spinless_view, records=deck_records)
         records[comp] = rec
         c, a = species.index(c_el), species.index(a_el)
         pv = np.array(rec["mb_pv"], dtype=np.float64)

@@ -16,13 +16,15 @@ BAND_NB = {mat.BANDS_10_NOSPIN: 10, mat.BANDS_20_SPIN: 20}
 
 class TBModel:
     def __init__(self, structure, compounds, band_model,
-                 passivate=True, db_shift=30.0, source=None, phi=None):
+                 passivate=True, db_shift=30.0, source=None, phi=None,
+                 deck_records=None):
         if band_model not in BAND_NB:
             raise ValueError("unsupported band model %s" % band_model)
         self.structure = structure
         self.band_model = band_model
         self.nb = BAND_NB[band_model]
-        self.tables = mat.param_init(compounds, band_model, source=source)
+        self.tables = mat.param_init(compounds, band_model, source=source,
+                                     deck_records=deck_records)
         self.passivate = bool(passivate)
         self.db_shift = float(db_shift)
         # per-atom electrostatic potential subtracted from the onsite

@@ -19,6 +19,7 @@ PTR_HOST, PTR_DEVICE = 0, 1
 SET_HV_STAGED = 1
 SET_RING_STEPS = 2
 SET_BLOCK_HV = 3
+SET_SK_ROLES = 4
 
 _STATUS = {0: "NH_OK", 1: "NH_ERR_ARG", 2: "NH_ERR_HIP", 3: "NH_ERR_STATE",
            4: "NH_ERR_NOMEM", 5: "NH_ERR_NOCONV"}
@@ -135,6 +136,17 @@ class NHTraceminOpts(ctypes.Structure):
     ]
 
 
+class NHVffParams(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_void_p), ("beta", ctypes.c_void_p),
+                ("a0", ctypes.c_void_p), ("aa", ctypes.c_void_p),
+                ("ba", ctypes.c_void_p), ("ca", ctypes.c_void_p),
+                ("model", ctypes.c_int32), ("fix_surface", ctypes.c_int32)]
+
+
+VFF_KEATING = 0
+VFF_KEATING_STRAINED = 1
+
+
 class NHEigResult(ctypes.Structure):
     _fields_ = [
         ("n_found", ctypes.c_int32),
@@ -183,6 +195,8 @@ SIGNATURES = {
     "nh_refine_eigenpairs": (ctypes.c_int, [P, P, ctypes.c_int32, P, ctypes.c_int,
                                             P, P, P, P]),
     "nh_vff_energy_grad": (ctypes.c_int, [P, P, P, P, P, P, P]),
+    "nh_vff_energy_grad_model": (ctypes.c_int, [P, P, P, P, P]),
+    "nh_vff_relax_model": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_double, P, P]),
     "nh_vff_relax": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_double,
                                     P, P]),
     "nh_cnn_create": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -284,3 +284,30 @@ def test_block_hv_variants_agree(case, precision, layout):
             assert err <= tol, (mode, c, err)
     same = 1e-15 if precision == "fp64" else 1e-7
     assert np.linalg.norm(out[1] - out[0]) <= same * np.linalg.norm(out[0])
+
+
+@pytest.mark.parametrize("layout", ["sk", "mf"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_sk_workgroup_roles_give_identical_products(layout, precision):
+    """Both workgroup mappings of the SK / matrix-free products (one role
+    per workgroup, or cations and anions of the same cells together) run
+    the same per-lane arithmetic: products are bit-identical; the Lanczos
+    tridiagonals agree to rounding (the workgroup partial sums of alpha
+    group the atoms differently)."""
+    from nemo3d_amd.solvers import eigensolve_lanczos
+    M = make("ingaas20_alloy_phi")
+    H = Hamiltonian(M, layout=native.LAYOUT_SK if layout == "sk" else native.LAYOUT_MATRIX_FREE,
+                    precision=native.PREC_FP64 if precision == "fp64" else native.PREC_FP32)
+    X = _models.random_vectors(H.n, 3, seed=71)
+    out, tri = {}, {}
+    for roles in (0, 1):
+        H.set(native.SET_SK_ROLES, roles)
+        out[roles] = H.hV(X, 0.3)
+        r = eigensolve_lanczos(H, num_vb=0, num_cb=1, erange_cb=(1e3, 1e3 + 1),
+                               max_iter=60, start_iter=10**6, want_vectors=False,
+                               return_tridiag=True)
+        tri[roles] = (r["alpha"], r["beta"])
+    H.set(native.SET_SK_ROLES, -1)
+    assert np.array_equal(out[0], out[1])
+    for k in (0, 1):
+        assert np.abs(tri[0][k] - tri[1][k]).max() <= 1e-10 * np.abs(tri[0][k]).max()

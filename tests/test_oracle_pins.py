@@ -145,3 +145,24 @@ def test_vff_energy_gives_keating_elastic_constants(kind):
                                                  + ev[0] * ev[2])
         # finite-strain terms are O(eps) relative
         assert abs(E / V - 8.0 / 3.0 * W) <= 5 * e0 * (8.0 / 3.0 * W), (kind, E / V, W)
+
+
+def test_strained_keating_reduces_to_keating():
+    """VFF_keating_strained with zero anharmonicity constants (A = B = C =
+    0: alpha_eff = alpha, beta_eff = the first leg's beta) is the harmonic
+    Keating energy on a binary crystal (both legs' beta equal), at random
+    displacements; with the deck's constants it differs."""
+    a = 0.56532
+    s = structure.binary_box((3, 3, 3), "Ga", "As", (a, a, a))
+    M = model.TBModel(s, ["GaAs"], materials.BANDS_20_SPIN)
+    s.disp = np.random.default_rng(4).normal(0, 0.01, (s.n_atoms, 3))
+    al, be, a0 = vff.keating_tables(M)
+    aa, ba, ca = vff.anharmonic_tables(M)
+    z = np.zeros_like(aa)
+    e_h = vff_oracle.energy(s, M.species, al, be, a0)
+    e_0 = vff_oracle.energy(s, M.species, al, be, a0, model="VFF_keating_strained",
+                            aa=z, ba=z + 1e-300, ca=z + 1e-300)
+    e_s = vff_oracle.energy(s, M.species, al, be, a0, model="VFF_keating_strained",
+                            aa=aa, ba=ba, ca=ca)
+    assert abs(e_0 - e_h) <= 1e-12 * e_h
+    assert abs(e_s - e_h) > 1e-3 * e_h
