@@ -65,3 +65,22 @@ def test_psi_squared_and_files(tmp_path):
     rows = np.loadtxt(tmp_path / "strain3d")
     assert rows.shape == (M.n_atoms, 9)
     assert np.loadtxt(tmp_path / "strain3d_cation").shape[0] == int(s.is_cation.sum())
+
+
+def test_psi_squared_matches_reference_wf_files(reference_root):
+    """writePsiSqr's output of the reference run (PostNEMO3D/bin/test/
+    silicon4.nd_wf_1..3, 64,000 atoms, 10 orbitals) from the same run's
+    eigenvectors (silicon4.nd_evec_1..3): equal to 1e-15."""
+    import os
+    t = os.path.join(reference_root, "PostNEMO3D/bin/test/")
+
+    def payload(path):
+        b = open(path, "rb").read()
+        return b[b.find(b"<\\HDR>") + 6:]
+
+    for k in (1, 2, 3):
+        wf = np.frombuffer(payload(t + "silicon4.nd_wf_%d" % k), "<f8")
+        ev = np.frombuffer(payload(t + "silicon4.nd_evec_%d" % k), "<f8").reshape(-1, 2)
+        p2 = post.psi_squared(ev[:, 0] + 1j * ev[:, 1], 10)
+        assert p2.shape == wf.shape
+        assert np.abs(p2 - wf).max() <= 1e-15, k
