@@ -138,10 +138,12 @@ inline bool use_staged(const nh_context *c) {
 }
 // SK / matrix-free products: both roles in one workgroup (waves 0-1 the
 // cations, waves 2-3 the anions at the same list position, so both rows of
-// a bond read its record within one workgroup) or one role per workgroup
-constexpr bool kSkMixedRolesDefault = false;
+// a bond read its record within one workgroup) or one role per workgroup.
+// Default: both roles with FP64 products, one role with FP32 (measured on
+// a 786,520-atom dot, fused Lanczos kernel: FP64 0.329 against 0.338 ms,
+// FP32 0.230 against 0.229 ms)
 inline bool sk_mixed_roles(const nh_context *c) {
-  return c->sk_roles < 0 ? kSkMixedRolesDefault : c->sk_roles != 0;
+  return c->sk_roles < 0 ? c->precision == NH_PREC_FP64 : c->sk_roles != 0;
 }
 }  // namespace nh
 
