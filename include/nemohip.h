@@ -250,6 +250,10 @@ typedef struct nh_eig_result {
   double *tridiag_alpha;
   double *tridiag_beta;
   int32_t tridiag_cap;
+  /* Lanczos: the same for the fused H·v kernel of the eigenvector pass
+     (it also applies the three-term step) */
+  double hv_ms_vec;
+  int32_t hv_timed_vec;
 } nh_eig_result;
 
 /* Lanczos without reorthogonalisation, eigenvalues from the tridiagonal

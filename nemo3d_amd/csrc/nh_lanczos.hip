@@ -576,19 +576,27 @@ class LanczosRun {
   std::vector<Converged> conv_;
   std::vector<double> lastRitz_[2];
   std::vector<char> lastConv_[2];
-  // sampled H·v timing: one event pair every kTimeEvery iterations
+  // sampled H·v timing: one event pair every kTimeEvery iterations, per
+  // pass (eigenvalue pass, eigenvector pass)
   static constexpr int kTimeEvery = 16;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_;
-  double hv_ms_sum_ = 0.0;
-  int hv_timed_ = 0;
+  struct Timed {
+    hipEvent_t start, stop;
+    bool value_pass;
+  };
+  std::vector<Timed> ev_;
+  double hv_ms_sum_[2] = {0.0, 0.0};
+  int hv_timed_[2] = {0, 0};
 
  public:
-  double hv_ms() const { return hv_timed_ ? hv_ms_sum_ / hv_timed_ : 0.0; }
-  int hv_timed() const { return hv_timed_; }
+  // pass 0: eigenvalue pass, 1: eigenvector pass
+  double hv_ms(int pass = 0) const {
+    return hv_timed_[pass] ? hv_ms_sum_[pass] / hv_timed_[pass] : 0.0;
+  }
+  int hv_timed(int pass = 0) const { return hv_timed_[pass]; }
   void release_events() {
     for (auto &e : ev_) {
-      hipEventDestroy(e.first);
-      hipEventDestroy(e.second);
+      hipEventDestroy(e.start);
+      hipEventDestroy(e.stop);
     }
     ev_.clear();
   }
@@ -598,9 +606,10 @@ nh_status LanczosRun::collect_timing() {
   NH_HIP(hipStreamSynchronize(c_->stream));
   for (auto &e : ev_) {
     float ms = 0.f;
-    NH_HIP(hipEventElapsedTime(&ms, e.first, e.second));
-    hv_ms_sum_ += ms;
-    ++hv_timed_;
+    NH_HIP(hipEventElapsedTime(&ms, e.start, e.stop));
+    const int pass = e.value_pass ? 0 : 1;
+    hv_ms_sum_[pass] += ms;
+    ++hv_timed_[pass];
   }
   release_events();
   return NH_OK;
@@ -713,14 +722,14 @@ nh_status LanczosRun::iterate(int k, bool red) {
   const double *bprev = k > 0 ? beta_ + (k - 1) : nullptr;
   const double *bprev2 = k > 1 ? beta_ + (k - 2) : nullptr;
   double *part = red ? part_ : nullptr;
-  const bool timed = red && (k % kTimeEvery) == 0;
+  const bool timed = (k % kTimeEvery) == 0;
   // the eigenvector pass knows alpha_k: the SK kernel applies the update
   const bool fuse = !red && sk_like(c);
-  std::pair<hipEvent_t, hipEvent_t> evp;
+  Timed evp{nullptr, nullptr, red};
   if (timed) {
-    NH_HIP(hipEventCreate(&evp.first));
-    NH_HIP(hipEventCreate(&evp.second));
-    NH_HIP(hipEventRecord(evp.first, c->stream));
+    NH_HIP(hipEventCreate(&evp.start));
+    NH_HIP(hipEventCreate(&evp.stop));
+    NH_HIP(hipEventRecord(evp.start, c->stream));
   }
 #define LZ_HV(NSP, TB, BLK)                                                    \
   do {                                                                         \
@@ -773,7 +782,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #undef LZ_SK1
 #undef LZ_SK2
   if (timed) {
-    NH_HIP(hipEventRecord(evp.second, c->stream));
+    NH_HIP(hipEventRecord(evp.stop, c->stream));
     ev_.push_back(evp);
   }
   if (red)
@@ -1157,8 +1166,10 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   NH_HIP(hipStreamSynchronize(c->stream));
   res->matvecs = matvecs;
   NH_TRY(collect_timing());
-  res->hv_ms = hv_ms();
-  res->hv_timed = hv_timed();
+  res->hv_ms = hv_ms(0);
+  res->hv_timed = hv_timed(0);
+  res->hv_ms_vec = hv_ms(1);
+  res->hv_timed_vec = hv_timed(1);
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return done ? NH_OK : NH_ERR_NOCONV;
 }

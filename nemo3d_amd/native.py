@@ -162,6 +162,8 @@ class NHEigResult(ctypes.Structure):
         ("tridiag_alpha", ctypes.c_void_p),
         ("tridiag_beta", ctypes.c_void_p),
         ("tridiag_cap", ctypes.c_int32),
+        ("hv_ms_vec", ctypes.c_double),
+        ("hv_timed_vec", ctypes.c_int32),
     ]
 
 

@@ -91,7 +91,8 @@ def eigensolve_lanczos(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
     out = {"eigenvalues": ev[:n].copy(), "residuals": rs[:n].copy(),
            "iterations": r.iterations, "matvecs": r.matvecs,
            "seconds": r.seconds, "converged": st == native.NH_OK,
-           "hv_ms": r.hv_ms, "hv_timed": r.hv_timed}
+           "hv_ms": r.hv_ms, "hv_timed": r.hv_timed,
+           "hv_ms_vec": r.hv_ms_vec, "hv_timed_vec": r.hv_timed_vec}
     if return_tridiag:
         out["alpha"] = ta[:r.iterations].copy()
         out["beta"] = tb[:r.iterations].copy()
@@ -342,6 +343,7 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
            "hv_ms": r1["hv_ms"], "hv_timed": r1["hv_timed"],
            "stage32": {k: r1[k] for k in ("iterations", "matvecs", "seconds",
                                           "converged", "hv_ms", "hv_timed",
+                                          "hv_ms_vec", "hv_timed_vec",
                                           "eigenvalues", "residuals")},
            "refine": {k: rf[k] for k in ("rounds", "matvecs", "seconds",
                                          "converged")}}
