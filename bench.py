@@ -66,13 +66,15 @@ def build_pyramid4m():
 
 
 # hnorm: spectral radius of H (Lanczos Ritz extremes -12.4 / 39.4 eV and
-# -12.8 / 38.4 eV); windows: end past the gap edges (found by a 9,000-step
-# Lanczos scan of each relaxed structure: dot 0.7230 / 1.4986 eV, pyramid
-# 0.5561 / 1.4516 eV) and hold more than 4 levels on each side
+# -12.8 / 38.4 eV); windows from a 9,000-step Lanczos scan of each relaxed
+# structure: the dot's hold exactly the 4 highest valence (0.7101-0.7230 eV;
+# the next is 0.7092) and the 4 lowest conduction levels (1.4986-1.5124;
+# the next is 1.5290), the pyramid's end past its gap edges (0.5561 /
+# 1.4516 eV) and hold 5 levels on each side
 WORKLOADS = {
     "dot786k": dict(
         build=build_dot786k, hnorm=40.0,
-        windows=dict(erange_vb=(0.64, 0.75), erange_cb=(1.47, 1.54)),
+        windows=dict(erange_vb=(0.7096, 0.7300), erange_cb=(1.4900, 1.5200)),
         text="InAs dome dot 20x5 nm + 1 ML WL in 30x30x20 nm GaAs"),
     "pyramid4m": dict(
         build=build_pyramid4m, hnorm=40.0,
@@ -330,8 +332,10 @@ def main():
                     help="Hamiltonian layout of the solves: stored SK records "
                          "or matrix-free (records recomputed per product)")
     ap.add_argument("--no-layout-probe", action="store_true")
-    ap.add_argument("--no-learned-routes", action="store_true",
-                    help="skip the learned-block block-Lanczos and TRACEMIN runs")
+    ap.add_argument("--learned-routes", action="store_true",
+                    help="also run the learned block through block Lanczos and "
+                         "PQ-TRACEMIN (minutes: measured 718 s and 41 s on the "
+                         "dot, neither converged within its cap)")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     hnorm, windows = wl["hnorm"], wl["windows"]
@@ -516,7 +520,7 @@ def main():
     if args.workload != "dot786k":
         cfg["lanczos_vector_memory"] = lanczos_memory(M.n)
     cfg["layout"] = {"sk": "stored SK records", "mf": "matrix-free"}[args.layout]
-    if not args.no_learned_routes and net is not None:
+    if args.learned_routes and net is not None:
         cfg["learned_block_routes"] = learned_routes(H, net, inputs, windows, hnorm)
     if not args.no_layout_probe:
         cfg["layout_comparison"] = layout_probe(M)

@@ -107,6 +107,11 @@ enum { NH_SET_BLOCK_HV = 3 };
  * (1 with FP64 products, 0 with FP32: the faster of the two measured).
  * The products are identical either way. */
 enum { NH_SET_SK_ROLES = 4 };
+/* NH_SET_SK_LANES: lanes per atom of the SK and matrix-free Lanczos
+ * products of 20-band models: 4 (a lane per bond, both spin blocks) or 8
+ * (a lane per bond and spin block: half the registers per lane); -1 the
+ * default (4). */
+enum { NH_SET_SK_LANES = 5 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 
 /* Assemble H on the device in the given layout and precision.  Replaces

@@ -47,17 +47,22 @@ struct SkLane {
   bool cat;
   int li;  // index into the cation or anion list
 };
+// LANES: lanes per atom (4: lane = bond; 8: lane = bond and spin, SkSplit)
+template <int LANES = 4>
 __device__ __forceinline__ SkLane sk_lane_of(SkGrid g, int b, int t) {
+  constexpr int kAtoms = kSkThreads / LANES;
   const int x = b & 7, s = b >> 3;
   if (g.mixed) {
     constexpr int kHalf = kSkThreads / 2;
-    return {t < kHalf, (x * g.ppx + s) * (kSkAtomsPerBlock / 2) + (t & (kHalf - 1)) / 4};
+    return {t < kHalf, (x * g.ppx + s) * (kAtoms / 2) + (t & (kHalf - 1)) / LANES};
   }
-  return {(s & 1) == 0, (x * g.ppx + (s >> 1)) * kSkAtomsPerBlock + t / 4};
+  return {(s & 1) == 0, (x * g.ppx + (s >> 1)) * kAtoms + t / LANES};
 }
 // the grid of a launch and its mapping
-inline SkGrid sk_grid(int n_cations, int n_anions, bool mixed, int *blocks) {
-  const int per = mixed ? kSkAtomsPerBlock / 2 : kSkAtomsPerBlock;
+inline SkGrid sk_grid(int n_cations, int n_anions, bool mixed, int *blocks,
+                      int lanes = 4) {
+  const int atoms = kSkThreads / lanes;
+  const int per = mixed ? atoms / 2 : atoms;
   const int nbc = (n_cations + per - 1) / per;
   const int nba = (n_anions + per - 1) / per;
   const int ppx = ((nbc > nba ? nbc : nba) + 7) / 8;
@@ -331,12 +336,16 @@ struct SkQuadStage {
   uint4 buf[kSkThreads * kStride];
 };
 
+// The NSP * 10 entries of an item start at entry `off` of the neighbour's
+// `stride` entries (stride = NSP * 10, off = 0: the whole atom; stride =
+// 20, off = 10 s: spin block s of a 20-band atom).
 template <int NSP, typename R, typename XT>
 __device__ __forceinline__ void sk_quad_gather(int j, int slot,
                                                const XT *__restrict__ x,
                                                SkQuadStage<NSP, XT> &L,
                                                R (&xr)[NSP][10],
-                                               R (&xi)[NSP][10]) {
+                                               R (&xi)[NSP][10],
+                                               int stride = NSP * 10, int off = 0) {
   using S = SkQuadStage<NSP, XT>;
   constexpr int NB = NSP * 10;
   constexpr int C = S::kChunks;
@@ -345,7 +354,7 @@ __device__ __forceinline__ void sk_quad_gather(int j, int slot,
   for (int n = 0; n < 4; ++n) {
     const int jn = __shfl(j, (t & 63 & ~3) | n, 64);
     if (jn >= 0) {
-      const uint4 *src = reinterpret_cast<const uint4 *>(x + (int64_t)jn * NB);
+      const uint4 *src = reinterpret_cast<const uint4 *>(x + (int64_t)jn * stride + off);
 #pragma unroll
       for (int c0 = 0; c0 < C; c0 += 4) {
         const int c = c0 + slot;
@@ -380,48 +389,50 @@ __device__ __forceinline__ void sk_quad_gather(int j, int slot,
 // rows slot, slot + 4, ... (64 contiguous bytes per quad and load), so that
 // the onsite couplings read their partner entries from LDS instead of
 // issuing scattered loads per row.
-template <int NSP>
+template <int NSP, int ATOMS = kSkAtomsPerBlock>
 struct SkOwnStage {
-  double2 x[kSkAtomsPerBlock][NSP * 10];
-  double2 os[kSkAtomsPerBlock][kOnsiteStride / 2];
+  double2 x[ATOMS][NSP * 10];
+  double2 os[ATOMS][kOnsiteStride / 2];
 };
 
 // Own entries and onsite record loaded into registers (before the barrier
 // that frees the staging LDS, so the loads overlap it) ...
-template <int NSP>
+// (LANES lanes per atom; lane `lane` takes entries lane, lane + LANES, ...)
+template <int NSP, int LANES = 4>
 struct SkOwnRegs {
-  static constexpr int kOs = (kOnsiteStride / 2 + 3) / 4;  // onsite chunks per lane
-  double2 x[(NSP * 10 + 3) / 4];
+  static constexpr int kOs = (kOnsiteStride / 2 + LANES - 1) / LANES;  // onsite chunks per lane
+  double2 x[(NSP * 10 + LANES - 1) / LANES];
   double2 os[kOs];
 };
-template <int NSP>
-__device__ __forceinline__ void sk_load_own(int slot, const double2 *__restrict__ xa,
+template <int NSP, int LANES = 4>
+__device__ __forceinline__ void sk_load_own(int lane, const double2 *__restrict__ xa,
                                             const double *__restrict__ os,
-                                            SkOwnRegs<NSP> &g) {
+                                            SkOwnRegs<NSP, LANES> &g) {
   constexpr int NB = NSP * 10;
 #pragma unroll
-  for (int i = 0; i < (NB + 3) / 4; ++i) {
-    const int r = 4 * i + slot;
+  for (int i = 0; i < (NB + LANES - 1) / LANES; ++i) {
+    const int r = LANES * i + lane;
     g.x[i] = r < NB ? xa[r] : make_double2(0.0, 0.0);
   }
   const double2 *o2 = reinterpret_cast<const double2 *>(os);  // 144-byte records
 #pragma unroll
-  for (int i = 0; i < SkOwnRegs<NSP>::kOs; ++i) {
-    const int c = 4 * i + slot;
+  for (int i = 0; i < SkOwnRegs<NSP, LANES>::kOs; ++i) {
+    const int c = LANES * i + lane;
     g.os[i] = c < kOnsiteStride / 2 ? o2[c] : make_double2(0.0, 0.0);
   }
 }
 // ... and stored to the epilogue staging after it
-template <int NSP>
-__device__ __forceinline__ void sk_store_own(int la, int slot, const SkOwnRegs<NSP> &g,
-                                             SkOwnStage<NSP> &S) {
+template <int NSP, int LANES = 4, int ATOMS = kSkAtomsPerBlock>
+__device__ __forceinline__ void sk_store_own(int la, int lane,
+                                             const SkOwnRegs<NSP, LANES> &g,
+                                             SkOwnStage<NSP, ATOMS> &S) {
   constexpr int NB = NSP * 10;
 #pragma unroll
-  for (int i = 0; i < (NB + 3) / 4; ++i)
-    if (4 * i + slot < NB) S.x[la][4 * i + slot] = g.x[i];
+  for (int i = 0; i < (NB + LANES - 1) / LANES; ++i)
+    if (LANES * i + lane < NB) S.x[la][LANES * i + lane] = g.x[i];
 #pragma unroll
-  for (int i = 0; i < SkOwnRegs<NSP>::kOs; ++i)
-    if (4 * i + slot < kOnsiteStride / 2) S.os[la][4 * i + slot] = g.os[i];
+  for (int i = 0; i < SkOwnRegs<NSP, LANES>::kOs; ++i)
+    if (LANES * i + lane < kOnsiteStride / 2) S.os[la][LANES * i + lane] = g.os[i];
 }
 
 // Exchange of a double with lane ^ 1 / lane ^ 2 inside the quad through
@@ -492,6 +503,53 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
         own_im[r >> 2] = im;
       }
     }
+}
+
+// SkSplit: 8 lanes per atom of a 20-band model, lane = (spin s, bond n):
+// lanes 4 s .. 4 s + 3 of an atom hold the four bonds for spin block s, so
+// each lane applies its bond's 10 x 10 block to one spin block only (half
+// the vector registers of sk_bonds), the record is read by both spin lanes
+// of a bond, and the four-lane sums stay within the spin group.  Lane
+// `slot` of spin group s keeps rows s * 10 + 4 i + slot (i < 3) in
+// own_re[i] / own_im[i].  QUAD only (LDS gather of the spin block).
+struct SkSplit {
+  static constexpr int kLanes = 8;
+  static constexpr int kAtoms = kSkThreads / kLanes;
+  static constexpr int kOwn = 3;  // rows per lane
+};
+
+template <typename R, bool CATION, typename XT, typename SRC>
+__device__ __forceinline__ void sk_bonds_split(int atom, int slot, int spin,
+                                               bool active, const SRC &src,
+                                               const int32_t *__restrict__ nbr,
+                                               const XT *__restrict__ xc,
+                                               double (&own_re)[SkSplit::kOwn],
+                                               double (&own_im)[SkSplit::kOwn],
+                                               SkQuadStage<1, XT> &L) {
+  R br[1][10], bi[1][10];
+#pragma unroll
+  for (int c = 0; c < 10; ++c) br[0][c] = bi[0][c] = R(0);
+  const int j = active ? nbr[atom * 4 + slot] : -1;
+  R xr[1][10], xi[1][10];
+  R rv[24];
+  if (SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
+  sk_quad_gather<1, R, XT>(j, slot, xc, L, xr, xi, 20, 10 * spin);
+  if (!SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
+  if (j >= 0)
+    BondApply<1, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3, xr,
+                                              xi, br, bi);
+#pragma unroll
+  for (int c = 0; c < 10; ++c) {
+    double re = double(br[0][c]), im = double(bi[0][c]);
+    re += quad_perm_f64<kQuadXor1>(re);
+    im += quad_perm_f64<kQuadXor1>(im);
+    re += quad_perm_f64<kQuadXor2>(re);
+    im += quad_perm_f64<kQuadXor2>(im);
+    if ((c & 3) == slot) {
+      own_re[c >> 2] = re;
+      own_im[c >> 2] = im;
+    }
+  }
 }
 
 // The same with the bond record already in registers (rv; j < 0: no bond)

@@ -89,6 +89,7 @@ struct nh_context {
   int ring_steps = 0;  // NH_SET_RING_STEPS (0: by free memory)
   int block_hv = -1;   // NH_SET_BLOCK_HV (-1: by measurement)
   int sk_roles = -1;   // NH_SET_SK_ROLES (-1: default, sk_mixed_roles)
+  int sk_lanes = -1;   // NH_SET_SK_LANES (-1: default, sk_split_lanes)
 
   // assembled Hamiltonian
   int layout = -1;
@@ -144,6 +145,11 @@ inline bool use_staged(const nh_context *c) {
 // FP32 0.230 against 0.229 ms)
 inline bool sk_mixed_roles(const nh_context *c) {
   return c->sk_roles < 0 ? c->precision == NH_PREC_FP64 : c->sk_roles != 0;
+}
+// SK / matrix-free products of 20-band models with the LDS gather: 8 lanes
+// per atom (one per bond and spin block) instead of 4 (one per bond)
+inline bool sk_split_lanes(const nh_context *c) {
+  return c->nb == 20 && use_staged(c) && c->sk_lanes == 8;
 }
 }  // namespace nh
 

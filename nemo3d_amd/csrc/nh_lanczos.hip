@@ -259,6 +259,97 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
   }
 }
 
+// k_lz_hv_sk with 8 lanes per atom of a 20-band model (SkSplit: lane =
+// spin block and bond), LDS gather; the same epilogue with lane (spin s,
+// slot) finishing rows s * 10 + 4 i + slot.  Fewer vector registers per
+// lane than k_lz_hv_sk (one spin block of the neighbour and of the
+// accumulators), so more waves fit on a SIMD.
+template <typename R, bool FUSE, typename SRC>
+__global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk_split(
+    int nc, const int32_t *__restrict__ cations, int na,
+    const int32_t *__restrict__ anions, SkGrid grid,
+    const double *__restrict__ onsite, const SRC src,
+    const int32_t *__restrict__ nbr, const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
+    double2 *__restrict__ Q, double2 *__restrict__ Y,
+    const double *__restrict__ beta_prev, double *__restrict__ part,
+    const double *__restrict__ alpha, const double2 *__restrict__ Rp,
+    const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
+  constexpr int NSP = 2, NB = 20;
+  constexpr int kLanes = SkSplit::kLanes, kAtoms = SkSplit::kAtoms;
+  const SkLane ln = sk_lane_of<kLanes>(grid, blockIdx.x, threadIdx.x);
+  // the role is uniform across the wavefront (scalar role branches)
+  const bool cat = __builtin_amdgcn_readfirstlane((int)ln.cat) != 0;
+  const int li = ln.li;
+  const int lane = threadIdx.x & (kLanes - 1), slot = lane & 3, spin = lane >> 2;
+  const bool active = li < (cat ? nc : na);
+  const int atom = active ? (cat ? cations[li] : anions[li]) : 0;
+  const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
+  const double al = FUSE ? alpha[0] : 0.0;
+  const double be = beta_prev ? beta_prev[0] : 0.0;
+  const double invp = beta_prev2 ? 1.0 / beta_prev2[0] : 1.0;
+  double own_re[SkSplit::kOwn], own_im[SkSplit::kOwn];
+  using XT = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+  const XT *xg;
+  if constexpr (sizeof(R) == 4) xg = Rv32;
+  else xg = Rv;
+  __shared__ SkQuadStage<1, XT> stage;
+  if (cat)
+    sk_bonds_split<R, true, XT>(atom, slot, spin, active, src, nbr, xg, own_re,
+                                own_im, stage);
+  else
+    sk_bonds_split<R, false, XT>(atom, slot, spin, active, src, nbr, xg, own_re,
+                                 own_im, stage);
+  static_assert(sizeof(SkOwnStage<NSP, kAtoms>) <= sizeof(SkQuadStage<1, XT>),
+                "epilogue staging must fit the gather staging");
+  auto &S = *reinterpret_cast<SkOwnStage<NSP, kAtoms> *>(stage.buf);
+  const int la = threadIdx.x / kLanes;
+  SkOwnRegs<NSP, kLanes> own;
+  if (active)
+    sk_load_own<NSP, kLanes>(lane, Rv + (int64_t)atom * NB,
+                             onsite + (size_t)atom * kOnsiteStride, own);
+  double2 rpv[SkSplit::kOwn];
+#pragma unroll
+  for (int i = 0; i < SkSplit::kOwn; ++i) rpv[i] = make_double2(0.0, 0.0);
+  if constexpr (FUSE) {
+    if (active && Rp) {
+#pragma unroll
+      for (int i = 0; i < SkSplit::kOwn; ++i)
+        if (4 * i + slot < 10) rpv[i] = Rp[(int64_t)atom * NB + spin * 10 + 4 * i + slot];
+    }
+  }
+  __syncthreads();  // every lane has read its staged neighbour entries
+  if (active) sk_store_own<NSP, kLanes, kAtoms>(la, lane, own, S);
+  __syncthreads();
+  double local = 0.0;
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < SkSplit::kOwn; ++i) {
+      const int c = 4 * i + slot;
+      if (c >= 10) break;
+      const int r = spin * 10 + c;
+      double re = own_re[i], im = own_im[i];
+      onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la], spin, c,
+                      0.0, re, im);
+      const double2 xo = S.x[la][r];
+      const double2 q = make_double2(xo.x * inv, xo.y * inv);
+      const int64_t e = (int64_t)atom * NB + r;
+      double2 y = make_double2(re * inv, im * inv);
+      if constexpr (FUSE) {
+        y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
+        if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+      }
+      if (Q) Q[e] = q;
+      Y[e] = y;
+      local += q.x * y.x + q.y * y.y;
+    }
+  }
+  if (part) {
+    double z = 0.0;
+    LzBlockSum<true>::run(local, z, &stage);
+    if (threadIdx.x == 0) part[blockIdx.x] = local;
+  }
+}
+
 // y <- y - alpha q - beta_prev q_prev with q = r * (1 / beta_prev) and
 // q_prev = r_prev * (1 / beta_prev2) formed on the fly (the same products
 // as the fused H·v kernel's q, so the trajectory does not depend on whether
@@ -698,7 +789,9 @@ nh_status LanczosRun::upload_start(float2 *f32) {
 nh_status LanczosRun::iterate(int k, bool red) {
   nh_context *c = c_;
   int sk_blocks = 0;
-  const SkGrid sg = sk_grid(c->n_cations, c->n_anions, sk_mixed_roles(c), &sk_blocks);
+  const bool split = sk_like(c) && sk_split_lanes(c);
+  const SkGrid sg = sk_grid(c->n_cations, c->n_anions, sk_mixed_roles(c), &sk_blocks,
+                            split ? SkSplit::kLanes : 4);
   const int gridA = sk_like(c)
                         ? sk_blocks
                         : (c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock;
@@ -753,9 +846,18 @@ nh_status LanczosRun::iterate(int k, bool red) {
     if (fuse) LZ_SK2(NSP, R, SRC, QUAD, true);                                  \
     else LZ_SK2(NSP, R, SRC, QUAD, false);                                      \
   } while (0)
+#define LZ_SPL2(R, SRC, FUSE)                                                    \
+  hipLaunchKernelGGL((k_lz_hv_sk_split<R, FUSE, decltype(SRC)>), dim3(gridA),   \
+                     dim3(kSkThreads), 0, c->stream, c->n_cations, c->cations,  \
+                     c->n_anions, c->anions, sg, c->onsite, SRC, c->nbr, Rk,    \
+                     g32, Q, Yk, bprev, part, alpha_ + k,                       \
+                     k > 0 ? Rkm1 : nullptr, bprev2, y32)
 #define LZ_SK0(NSP, R, SRC)                                                     \
   do {                                                                          \
-    if (use_staged(c)) LZ_SK1(NSP, R, SRC, true);                                \
+    if (NSP == 2 && split) {                                                    \
+      if (fuse) LZ_SPL2(R, SRC, true);                                          \
+      else LZ_SPL2(R, SRC, false);                                              \
+    } else if (use_staged(c)) LZ_SK1(NSP, R, SRC, true);                        \
     else LZ_SK1(NSP, R, SRC, false);                                            \
   } while (0)
 #define LZ_SK(R, SRC)                                                           \
@@ -781,6 +883,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #undef LZ_SK0
 #undef LZ_SK1
 #undef LZ_SK2
+#undef LZ_SPL2
   if (timed) {
     NH_HIP(hipEventRecord(evp.stop, c->stream));
     ev_.push_back(evp);

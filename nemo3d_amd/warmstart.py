@@ -9,6 +9,10 @@ eigensolve_lanczos), each with its Kramers partner; the network learns to
 span each level's two-dimensional subspace.
 
     python -m nemo3d_amd.warmstart train --dots 16 --out FILE.npz
+
+A larger training set is built in rounds: --first K takes the dots K,
+K + 1, ... of the seeded sequence, --resume FILE continues from a saved
+network (its weights; Adam restarts) and keeps its list of dots.
 """
 import argparse
 import ctypes
@@ -208,9 +212,13 @@ def main():
     ap.add_argument("--cb-window", type=float, nargs=2, default=(1.0, 1.8),
                     help="conduction window (eV); its bottom lies in the gap")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--first", type=int, default=0,
+                    help="index of the first dot of the seeded sequence")
+    ap.add_argument("--resume", default=None,
+                    help="continue from this saved network")
     a = ap.parse_args()
     samples, meta = [], []
-    for k in range(a.dots):
+    for k in range(a.first, a.first + a.dots):
         p = training_params(k)
         t0 = time.time()
         M = dot_model(tuple(a.box), p["base_nm"], p["height_nm"], p["x_in"], p["seed"])
@@ -234,10 +242,17 @@ def main():
                          lanczos_iter=r["iterations"]))
         print("dot %d: %d atoms, levels %s, %.1fs" % (
             k, M.n_atoms, np.round(r["eigenvalues"], 4), time.time() - t0), flush=True)
-    net = new_network(samples[0]["owner"], samples[0]["feats"].shape[1])
+    prev = {}
+    if a.resume:
+        net, prev = load(samples[0]["owner"], a.resume)
+    else:
+        net = new_network(samples[0]["owner"], samples[0]["feats"].shape[1])
     hist = train(net, samples, a.epochs, a.lr,
                  log=lambda m: print(m, flush=True))
-    save(net, a.out, dict(dots=meta, loss=hist, box_nm=list(a.box)))
+    save(net, a.out, dict(dots=prev.get("dots", []) + meta,
+                          loss=prev.get("loss", []) + hist, box_nm=list(a.box),
+                          rounds=prev.get("rounds", []) + [
+                              dict(first=a.first, dots=len(meta), epochs=a.epochs)]))
 
 
 if __name__ == "__main__":
