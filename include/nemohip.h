@@ -110,7 +110,8 @@ enum { NH_SET_SK_ROLES = 4 };
 /* NH_SET_SK_LANES: lanes per atom of the SK and matrix-free Lanczos
  * products of 20-band models: 4 (a lane per bond, both spin blocks) or 8
  * (a lane per bond and spin block: half the registers per lane); -1 the
- * default (4). */
+ * default (8 for matrix-free FP32 products, 4 otherwise: the faster of the
+ * two measured). */
 enum { NH_SET_SK_LANES = 5 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 
