@@ -465,7 +465,8 @@ def main():
         passes = [("eigenvalue pass", lz_bytes(True), st["hv_ms"], st["iterations"],
                    None if mf else "k_lz_hv_sk_fp32"),
                   ("eigenvector pass", None if mf else sk_lanczos_vec_bytes(M, 4, True),
-                   st["hv_ms_vec"], st["matvecs"] - st["iterations"], None)]
+                   st["hv_ms_vec"], st["matvecs"] - st["iterations"],
+                   None if mf else "k_lz_hv_sk_fp32_vec")]
     else:
         prec, p32 = "FP64", False
         passes = [("eigenvalue pass", lz_bytes(False), plain["hv_ms"], plain["iterations"],

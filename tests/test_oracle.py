@@ -299,3 +299,19 @@ def test_mixed_species_onsite_is_the_neighbour_mean(band):
     # interior and surface anions with mixed neighbourhoods were seen
     assert {(4, 1), (4, 2), (4, 3)} <= checked
     assert any(n < 4 and 0 < k < n for n, k in checked)
+
+
+def test_tracemin_example_hamiltonian():
+    """The reference's TRACEMIN example Hamiltonian (TRACEMIN/Examples/
+    pqexample2_full, a 960 x 960 matrix NEMO-3D wrote for the example:
+    48-atom periodic Si, 20-band + spin-orbit, printed to 13 significant
+    digits): the oracle's assembly of the same structure equals it entry
+    for entry to the print precision, SK couplings, spin-orbit pairs,
+    periodic neighbours and row order included."""
+    import _models
+    M = _models.pq_si48()
+    Hd = oracle.assemble(M.arrays()).dense()
+    Hc = _models.pq_example_matrix()
+    assert Hd.shape == Hc.shape == (960, 960)
+    assert np.abs(Hd - Hc).max() <= 5e-12
+    assert np.count_nonzero(np.abs(Hc) > 0) == 34560
