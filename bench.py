@@ -111,29 +111,36 @@ def lanczos_memory(n, free_bytes=288e9, k=8):
             "eigenvector_pass_fp32_ring_steps": k32}
 
 
-def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False):
+def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False, fp32_vectors=False):
     """Algorithmic bytes of one fused Lanczos H·v launch of the eigenvalue
     pass in the SK layout: onsite record (18 doubles) per atom, one 24-value
     record per bond (each bond once; rec_bytes per value), neighbour and
     bond indices (4 + 4 int32 per atom), the atom lists, x read once (FP64)
     and y written once (q is stored only in the eigenvector pass); with
     FP32 products the neighbours' entries come from the FP32 copy of x
-    (8 bytes per entry, read once)."""
+    (8 bytes per entry, read once).  fp32_vectors: the Lanczos vectors are
+    FP32, so x is read once as FP32 (own entries and neighbours) and y
+    (FP64) written once."""
     A = M.n_atoms
     nb = int((M.structure.nbr >= 0).sum()) // 2
     mat = A * 18 * 8 + nb * 24 * rec_bytes + A * 4 * 4 * 2 + A * 4
+    if fp32_vectors:
+        return mat + M.n * (8 + 16)
     return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
-def sk_lanczos_vec_bytes(M, rec_bytes=8, fp32_copy=False):
+def sk_lanczos_vec_bytes(M, rec_bytes=8, fp32_copy=False, fp32_vectors=False):
     """Algorithmic bytes of one fused H*v launch of the eigenvector pass
     (SK layout), which also applies the three-term step: the matrix bytes
     of sk_lanczos_bytes, r_k read once (FP64), r_(k-1) read and r_(k+1)
     written (FP64); with FP32 products the neighbours' entries come from
-    the FP32 copy of r_k and the FP32 copy of r_(k+1) is written."""
+    the FP32 copy of r_k and the FP32 copy of r_(k+1) is written.
+    fp32_vectors: r_k, r_(k-1) read and r_(k+1) written in FP32 only."""
     A = M.n_atoms
     nb = int((M.structure.nbr >= 0).sum()) // 2
     mat = A * 18 * 8 + nb * 24 * rec_bytes + A * 4 * 4 * 2 + A * 4
+    if fp32_vectors:
+        return mat + M.n * 3 * 8
     return mat + 3 * M.n * 16 + (2 * M.n * 8 if fp32_copy else 0)
 
 
@@ -220,14 +227,17 @@ def cpu_model():
     return "unknown"
 
 
-def mf_lanczos_bytes(M, fp32_copy=False):
+def mf_lanczos_bytes(M, fp32_copy=False, fp32_vectors=False):
     """Algorithmic bytes of one fused Lanczos H*v launch in the matrix-free
     layout: onsite record (18 doubles), displacement + species (4 doubles),
     the packed ideal bond vectors and the neighbour indices (4 + 4 int32)
     and the list entry per atom, x read once (FP64), y written once, and
-    with FP32 products the FP32 copy of x (read once)."""
+    with FP32 products the FP32 copy of x (read once); fp32_vectors: x read
+    once as FP32, y (FP64) written once."""
     A = M.n_atoms
     mat = A * (18 * 8 + 4 * 8 + 4 * 4 * 2 + 4)
+    if fp32_vectors:
+        return mat + M.n * (8 + 16)
     return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
@@ -332,6 +342,9 @@ def main():
                     help="Hamiltonian layout of the solves: stored SK records "
                          "or matrix-free (records recomputed per product)")
     ap.add_argument("--no-layout-probe", action="store_true")
+    ap.add_argument("--lz-vectors", choices=["fp64", "fp32"], default="fp32",
+                    help="storage of the Lanczos vectors of the FP32-product stage "
+                         "(the three-term step is FP64 either way)")
     ap.add_argument("--learned-routes", action="store_true",
                     help="also run the learned block through block Lanczos and "
                          "PQ-TRACEMIN (minutes: measured 718 s and 41 s on the "
@@ -356,6 +369,8 @@ def main():
     H = Hamiltonian(M, layout=layout, precision=native.PREC_FP64)
     H.set_precision(native.PREC_FP32)  # FP32 copies made outside the timing
     H.set_precision(native.PREC_FP64)
+    v32 = args.lz_vectors == "fp32"
+    H.set(native.SET_LZ_VECTORS, 1 if v32 else 0)
     setup["assembly_s"] = time.time() - t1
     net = inputs = None
     if args.form != "plain" and os.path.exists(args.weights):
@@ -453,8 +468,9 @@ def main():
 
     def lz_bytes(fp32):
         if mf:
-            return mf_lanczos_bytes(M, fp32_copy=fp32)
-        return sk_lanczos_bytes(M, rec_bytes=4, fp32_copy=True) if fp32 else sk_lanczos_bytes(M)
+            return mf_lanczos_bytes(M, fp32_copy=fp32, fp32_vectors=fp32 and v32)
+        return (sk_lanczos_bytes(M, rec_bytes=4, fp32_copy=True, fp32_vectors=v32) if fp32
+                else sk_lanczos_bytes(M))
 
     # the two fused H*v kernels of the headline form (eigenvalue pass and
     # eigenvector pass, which also applies the three-term step); `roofline`
@@ -463,10 +479,10 @@ def main():
         st = mixed["stage32"]
         prec, p32 = "FP32 products", True
         passes = [("eigenvalue pass", lz_bytes(True), st["hv_ms"], st["iterations"],
-                   None if mf else "k_lz_hv_sk_fp32"),
-                  ("eigenvector pass", None if mf else sk_lanczos_vec_bytes(M, 4, True),
+                   None if mf else "k_lz_hv_sk_fp32" + ("_v32" if v32 else "")),
+                  ("eigenvector pass", None if mf else sk_lanczos_vec_bytes(M, 4, True, v32),
                    st["hv_ms_vec"], st["matvecs"] - st["iterations"],
-                   None if mf else "k_lz_hv_sk_fp32_vec")]
+                   None if mf else "k_lz_hv_sk_fp32_vec" + ("_v32" if v32 else ""))]
     else:
         prec, p32 = "FP64", False
         passes = [("eigenvalue pass", lz_bytes(False), plain["hv_ms"], plain["iterations"],
@@ -480,9 +496,10 @@ def main():
         ach = b / (ms * 1e-3) / 1e9
         rooflines.append({
             "bound": "hbm",
-            "kernel": "k_lz_hv_sk<2,%s%s> (fused Lanczos H*v, %s, %s %s)" % (
+            "kernel": "k_lz_hv_sk<2,%s%s%s> (fused Lanczos H*v, %s, %s %s%s)" % (
                 "float" if p32 else "double", ",FUSE" if pname == "eigenvector pass" else "",
-                pname, lname, prec),
+                ",float2" if (p32 and v32) else "", pname, lname, prec,
+                ", FP32 Lanczos vectors" if (p32 and v32) else ""),
             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
             # measured (PMC) on the default workload only
@@ -521,6 +538,7 @@ def main():
     if args.workload != "dot786k":
         cfg["lanczos_vector_memory"] = lanczos_memory(M.n)
     cfg["layout"] = {"sk": "stored SK records", "mf": "matrix-free"}[args.layout]
+    cfg["lanczos_vectors_fp32_stage"] = args.lz_vectors
     if args.learned_routes and net is not None:
         cfg["learned_block_routes"] = learned_routes(H, net, inputs, windows, hnorm)
     if not args.no_layout_probe:

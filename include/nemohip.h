@@ -113,6 +113,14 @@ enum { NH_SET_SK_ROLES = 4 };
  * default (8 for matrix-free FP32 products, 4 otherwise: the faster of the
  * two measured). */
 enum { NH_SET_SK_LANES = 5 };
+/* NH_SET_LZ_VECTORS: storage of the Lanczos vectors r_k of a Lanczos run
+ * with FP32 SK or matrix-free products: 0 FP64 (with an FP32 copy for the
+ * products' gathers), 1 FP32 (the three-term step still computed in FP64,
+ * its result rounded once; no FP64 vector in either pass), -1 the default
+ * (1: measured as fast or faster, with the same number of steps).  Either
+ * way the eigenvector pass retraces the eigenvalue pass bit for bit; other
+ * layouts and FP64 products always keep FP64 vectors. */
+enum { NH_SET_LZ_VECTORS = 6 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 
 /* Assemble H on the device in the given layout and precision.  Replaces

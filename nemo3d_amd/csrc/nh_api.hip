@@ -328,6 +328,11 @@ nh_status nh_context_set(nh_context *ctx, int what, int value) {
                "NH_SET_SK_LANES: value %d not -1, 4 or 8", value);
       ctx->sk_lanes = value;
       return NH_OK;
+    case NH_SET_LZ_VECTORS:
+      NH_CHECK(value >= -1 && value <= 1, NH_ERR_ARG,
+               "NH_SET_LZ_VECTORS: value %d not in -1..1", value);
+      ctx->lz_vectors = value;
+      return NH_OK;
     case NH_SET_SK_ROLES:
       NH_CHECK(value >= -1 && value <= 1, NH_ERR_ARG,
                "NH_SET_SK_ROLES: value %d not in -1..1", value);

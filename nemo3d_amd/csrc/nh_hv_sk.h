@@ -70,14 +70,18 @@ inline SkGrid sk_grid(int n_cations, int n_anions, bool mixed, int *blocks,
   return SkGrid{ppx, mixed ? 1 : 0};
 }
 
+// Complex entries of the FP64 vector, or of an FP32 one widened on load
+__device__ __forceinline__ double2 widen2(double2 v) { return v; }
+__device__ __forceinline__ double2 widen2(float2 v) { return make_double2(v.x, v.y); }
+
 // Onsite contribution (diagonal - shift, surface s-p couplings, spin-orbit)
 // to row (s, o) of atom with vector base pointer xa.
-template <int NSP>
+template <int NSP, typename XA = double2>
 __device__ __forceinline__ void onsite_row(const double *__restrict__ os,
-                                           const double2 *__restrict__ xa,
+                                           const XA *__restrict__ xa,
                                            int s, int o, double shift,
                                            double &re, double &im) {
-  const double2 xo = xa[s * 10 + o];
+  const double2 xo = widen2(xa[s * 10 + o]);
   const double d = os[o] - shift;
   re += d * xo.x;
   im += d * xo.y;
@@ -87,7 +91,7 @@ __device__ __forceinline__ void onsite_row(const double *__restrict__ os,
     const int p1 = o <= 2 ? 3 : 2, k1 = o == 1 ? 1 : o == 2 ? 3 : o == 3 ? 3 : 4;
     const int p2 = o <= 3 ? 4 : 3, k2 = o == 1 ? 2 : o == 2 ? 4 : 5;
     const double h0 = os[10 + k0], h1 = os[10 + k1], h2 = os[10 + k2];
-    const double2 a0 = xa[s * 10 + p0], a1 = xa[s * 10 + p1], a2 = xa[s * 10 + p2];
+    const double2 a0 = widen2(xa[s * 10 + p0]), a1 = widen2(xa[s * 10 + p1]), a2 = widen2(xa[s * 10 + p2]);
     re += h0 * a0.x + h1 * a1.x + h2 * a2.x;
     im += h0 * a0.y + h1 * a1.y + h2 * a2.y;
   }
@@ -95,25 +99,25 @@ __device__ __forceinline__ void onsite_row(const double *__restrict__ os,
     const double lam = os[16];
     if (s == 0) {
       if (o == 2) {
-        re += lam * xa[3].y + lam * xa[14].x;
-        im += -lam * xa[3].x + lam * xa[14].y;
+        re += lam * widen2(xa[3]).y + lam * widen2(xa[14]).x;
+        im += -lam * widen2(xa[3]).x + lam * widen2(xa[14]).y;
       } else if (o == 3) {
-        re += -lam * xa[2].y + lam * xa[14].y;
-        im += lam * xa[2].x - lam * xa[14].x;
+        re += -lam * widen2(xa[2]).y + lam * widen2(xa[14]).y;
+        im += lam * widen2(xa[2]).x - lam * widen2(xa[14]).x;
       } else {
-        re += -lam * xa[12].x - lam * xa[13].y;
-        im += -lam * xa[12].y + lam * xa[13].x;
+        re += -lam * widen2(xa[12]).x - lam * widen2(xa[13]).y;
+        im += -lam * widen2(xa[12]).y + lam * widen2(xa[13]).x;
       }
     } else {
       if (o == 2) {
-        re += -lam * xa[13].y - lam * xa[4].x;
-        im += lam * xa[13].x - lam * xa[4].y;
+        re += -lam * widen2(xa[13]).y - lam * widen2(xa[4]).x;
+        im += lam * widen2(xa[13]).x - lam * widen2(xa[4]).y;
       } else if (o == 3) {
-        re += lam * xa[12].y + lam * xa[4].y;
-        im += -lam * xa[12].x - lam * xa[4].x;
+        re += lam * widen2(xa[12]).y + lam * widen2(xa[4]).y;
+        im += -lam * widen2(xa[12]).x - lam * widen2(xa[4]).x;
       } else {
-        re += lam * xa[2].x - lam * xa[3].y;
-        im += lam * xa[2].y + lam * xa[3].x;
+        re += lam * widen2(xa[2]).x - lam * widen2(xa[3]).y;
+        im += lam * widen2(xa[2]).y + lam * widen2(xa[3]).x;
       }
     }
   }
@@ -404,15 +408,16 @@ struct SkOwnRegs {
   double2 x[(NSP * 10 + LANES - 1) / LANES];
   double2 os[kOs];
 };
-template <int NSP, int LANES = 4>
-__device__ __forceinline__ void sk_load_own(int lane, const double2 *__restrict__ xa,
+// (xa: the FP64 vector, or the FP32 one whose entries are widened on load)
+template <int NSP, int LANES = 4, typename XO = double2>
+__device__ __forceinline__ void sk_load_own(int lane, const XO *__restrict__ xa,
                                             const double *__restrict__ os,
                                             SkOwnRegs<NSP, LANES> &g) {
   constexpr int NB = NSP * 10;
 #pragma unroll
   for (int i = 0; i < (NB + LANES - 1) / LANES; ++i) {
     const int r = LANES * i + lane;
-    g.x[i] = r < NB ? xa[r] : make_double2(0.0, 0.0);
+    g.x[i] = r < NB ? widen2(xa[r]) : make_double2(0.0, 0.0);
   }
   const double2 *o2 = reinterpret_cast<const double2 *>(os);  // 144-byte records
 #pragma unroll

@@ -132,25 +132,34 @@ __device__ __forceinline__ double2 lz_three_term_v(double2 y, double2 q,
   }
   return y;
 }
-// the same step with r_prev read from rp (null: first step)
+// the same step with r_prev read from rp (null: first step; FP64 or FP32
+// storage)
+template <typename XO>
 __device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
-                                                 const double2 *rp, double al,
+                                                 const XO *rp, double al,
                                                  double be, double invp) {
-  return lz_three_term_v(y, q, rp != nullptr, rp ? *rp : make_double2(0.0, 0.0),
-                         al, be, invp);
+  return lz_three_term_v(y, q, rp != nullptr,
+                         rp ? widen2(*rp) : make_double2(0.0, 0.0), al, be, invp);
 }
 
-template <int NSP, typename R, bool QUAD, bool FUSE, typename SRC>
+// XO: storage of the Lanczos vectors r.  double2: FP64 r (own entries,
+// r_prev) with an FP32 copy for the gathers of FP32 products; float2 (FP32
+// products only): the r vectors themselves are FP32 (Rv == Rv32), the
+// eigenvector pass writes the rounded r_{k+1} to Y32 only, and the step's
+// arithmetic stays FP64.
+template <int NSP, typename R, bool QUAD, bool FUSE, typename SRC, typename XO = double2>
 __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, SkGrid grid,
     const double *__restrict__ onsite, const SRC src,
-    const int32_t *__restrict__ nbr, const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
+    const int32_t *__restrict__ nbr, const XO *__restrict__ Rv, const float2 *__restrict__ Rv32,
     double2 *__restrict__ Q, double2 *__restrict__ Y,
     const double *__restrict__ beta_prev, double *__restrict__ part,
-    const double *__restrict__ alpha, const double2 *__restrict__ Rp,
+    const double *__restrict__ alpha, const XO *__restrict__ Rp,
     const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
   constexpr int NB = NSP * 10;
+  constexpr bool V32 = std::is_same<XO, float2>::value;
+  static_assert(!V32 || sizeof(R) == 4, "FP32 Lanczos vectors need FP32 products");
   constexpr int NO = (NB + 3) / 4;
   const SkLane ln = sk_lane_of(grid, blockIdx.x, threadIdx.x);
   // the role is uniform across the wavefront; read as a scalar, the role
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
       if (active && Rp) {
 #pragma unroll
         for (int i = 0; i < NO; ++i)
-          if (4 * i + slot < NB) rpv[i] = Rp[(int64_t)atom * NB + 4 * i + slot];
+          if (4 * i + slot < NB) rpv[i] = widen2(Rp[(int64_t)atom * NB + 4 * i + slot]);
       }
     }
     __syncthreads();  // every lane has read its staged neighbour entries
@@ -223,32 +232,32 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
         double2 y = make_double2(re * inv, im * inv);
         if constexpr (FUSE) {
           y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
-          if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+          if (V32 || Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
         }
         if (Q) Q[e] = q;
-        Y[e] = y;
+        if constexpr (!(V32 && FUSE)) Y[e] = y;
         local += q.x * y.x + q.y * y.y;
       }
     }
   } else if (active) {
     const double *os = onsite + (size_t)atom * kOnsiteStride;
-    const double2 *xa = Rv + (int64_t)atom * NB;
+    const XO *xa = Rv + (int64_t)atom * NB;
 #pragma unroll
     for (int i = 0; i < NO; ++i) {
       const int r = 4 * i + slot;
       if (r >= NB) break;
       double re = own_re[i], im = own_im[i];
       onsite_row<NSP>(os, xa, r / 10, r % 10, 0.0, re, im);
-      const double2 xr = xa[r];
+      const double2 xr = widen2(xa[r]);
       const double2 q = make_double2(xr.x * inv, xr.y * inv);
       const int64_t e = (int64_t)atom * NB + r;
       double2 y = make_double2(re * inv, im * inv);
       if constexpr (FUSE) {
         y = lz_three_term(y, q, Rp ? Rp + e : nullptr, al, be, invp);
-        if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+        if (V32 || Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
       }
       if (Q) Q[e] = q;
-      Y[e] = y;
+      if constexpr (!(V32 && FUSE)) Y[e] = y;
       local += q.x * y.x + q.y * y.y;
     }
   }
@@ -264,17 +273,19 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
 // slot) finishing rows s * 10 + 4 i + slot.  Fewer vector registers per
 // lane than k_lz_hv_sk (one spin block of the neighbour and of the
 // accumulators), so more waves fit on a SIMD.
-template <typename R, bool FUSE, typename SRC>
+template <typename R, bool FUSE, typename SRC, typename XO = double2>
 __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk_split(
     int nc, const int32_t *__restrict__ cations, int na,
     const int32_t *__restrict__ anions, SkGrid grid,
     const double *__restrict__ onsite, const SRC src,
-    const int32_t *__restrict__ nbr, const double2 *__restrict__ Rv, const float2 *__restrict__ Rv32,
+    const int32_t *__restrict__ nbr, const XO *__restrict__ Rv, const float2 *__restrict__ Rv32,
     double2 *__restrict__ Q, double2 *__restrict__ Y,
     const double *__restrict__ beta_prev, double *__restrict__ part,
-    const double *__restrict__ alpha, const double2 *__restrict__ Rp,
+    const double *__restrict__ alpha, const XO *__restrict__ Rp,
     const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
   constexpr int NSP = 2, NB = 20;
+  constexpr bool V32 = std::is_same<XO, float2>::value;
+  static_assert(!V32 || sizeof(R) == 4, "FP32 Lanczos vectors need FP32 products");
   constexpr int kLanes = SkSplit::kLanes, kAtoms = SkSplit::kAtoms;
   const SkLane ln = sk_lane_of<kLanes>(grid, blockIdx.x, threadIdx.x);
   // the role is uniform across the wavefront (scalar role branches)
@@ -314,7 +325,8 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
     if (active && Rp) {
 #pragma unroll
       for (int i = 0; i < SkSplit::kOwn; ++i)
-        if (4 * i + slot < 10) rpv[i] = Rp[(int64_t)atom * NB + spin * 10 + 4 * i + slot];
+        if (4 * i + slot < 10)
+          rpv[i] = widen2(Rp[(int64_t)atom * NB + spin * 10 + 4 * i + slot]);
     }
   }
   __syncthreads();  // every lane has read its staged neighbour entries
@@ -336,10 +348,10 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
       double2 y = make_double2(re * inv, im * inv);
       if constexpr (FUSE) {
         y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
-        if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+        if (V32 || Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
       }
       if (Q) Q[e] = q;
-      Y[e] = y;
+      if constexpr (!(V32 && FUSE)) Y[e] = y;
       local += q.x * y.x + q.y * y.y;
     }
   }
@@ -356,11 +368,16 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
 // q was stored), partial sums of |y|^2.  Rp may be null (first step), and
 // beta_prev / beta_prev2 null mean 1 (the start vector is normalised).
 // Y32 (optional): FP32 copy of the new r for the next FP32 product.
+// XO = float2: r_k, r_prev are FP32 and the rounded r_{k+1} goes to Y32
+// only (Y keeps H q); |r|^2 is summed over the rounded entries, so that
+// r_{k+1} / beta_k has unit norm.
+template <typename XO>
 __global__ __launch_bounds__(kRedThreads) void k_lz_update(
-    int64_t n, double2 *__restrict__ Y, const double2 *__restrict__ Rk,
-    const double2 *__restrict__ Rp, const double *__restrict__ alpha,
+    int64_t n, double2 *__restrict__ Y, const XO *__restrict__ Rk,
+    const XO *__restrict__ Rp, const double *__restrict__ alpha,
     const double *__restrict__ beta_prev, const double *__restrict__ beta_prev2,
     double *__restrict__ part, float2 *__restrict__ Y32) {
+  constexpr bool V32 = std::is_same<XO, float2>::value;
   const double al = alpha[0];
   const double be = beta_prev ? beta_prev[0] : 0.0;
   const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
@@ -368,12 +385,18 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_update(
   double s = 0.0, z = 0.0;
   for (int64_t k = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * kRedThreads) {
-    const double2 r = Rk[k];
+    const double2 r = widen2(Rk[k]);
     const double2 q = make_double2(r.x * inv, r.y * inv);
     const double2 y = lz_three_term(Y[k], q, Rp ? Rp + k : nullptr, al, be, invp);
-    Y[k] = y;
-    if (Y32) Y32[k] = make_float2((float)y.x, (float)y.y);
-    s += y.x * y.x + y.y * y.y;
+    if constexpr (V32) {
+      const float2 f = make_float2((float)y.x, (float)y.y);
+      Y32[k] = f;
+      s += (double)f.x * f.x + (double)f.y * f.y;
+    } else {
+      Y[k] = y;
+      if (Y32) Y32[k] = make_float2((float)y.x, (float)y.y);
+      s += y.x * y.x + y.y * y.y;
+    }
   }
   if (part) {
     block_sum2(s, z);
@@ -641,6 +664,11 @@ class LanczosRun {
   double2 *R_ = nullptr, *Y_ = nullptr, *Rm1_ = nullptr, *ring_ = nullptr;
   // FP32 copies of r for the FP32 SK products (null otherwise)
   float2 *R32_ = nullptr, *Y32_ = nullptr;
+  // FP32 Lanczos vectors (lz_fp32_vectors): r_k, r_{k-1}, r_{k+1} of the
+  // eigenvalue pass are R32_, Rm132_, Y32_; Y_ receives H q; R_ and Rm1_
+  // are not allocated
+  bool v32_ = false;
+  float2 *Rm132_ = nullptr;
   // ring length of the eigenvector pass (<= kRingMax)
   int ring_k_ = kRingMax;
   // eigenvector pass with FP32 products: 2 K FP32 copies of r, slot
@@ -774,13 +802,15 @@ nh_status LanczosRun::build_start(const nh_lanczos_opts *o) {
   return NH_OK;
 }
 
-// r <- r0 (and its FP32 copy f32 when the products are FP32)
+// r <- r0 (and its FP32 copy f32 when the products are FP32; only the
+// latter with FP32 Lanczos vectors)
 nh_status LanczosRun::upload_start(float2 *f32) {
-  NH_HIP(hipMemcpyAsync(R_, R0_, n_ * sizeof(double2), hipMemcpyDeviceToDevice,
-                        c_->stream));
+  if (!v32_)
+    NH_HIP(hipMemcpyAsync(R_, R0_, n_ * sizeof(double2), hipMemcpyDeviceToDevice,
+                          c_->stream));
   if (f32) {
     hipLaunchKernelGGL(k_to_f32, dim3(reduce_blocks(n_)), dim3(256), 0,
-                       c_->stream, n_, R_, f32);
+                       c_->stream, n_, R0_, f32);
     NH_HIP(hipGetLastError());
   }
   return NH_OK;
@@ -818,6 +848,20 @@ nh_status LanczosRun::iterate(int k, bool red) {
   const bool timed = (k % kTimeEvery) == 0;
   // the eigenvector pass knows alpha_k: the SK kernel applies the update
   const bool fuse = !red && sk_like(c);
+  // FP32 Lanczos vectors: own entries and gathers from r_k (FP32), r_{k-1}
+  // from the previous FP32 slot; the eigenvector pass writes only the FP32
+  // r_{k+1} (no FP64 vector at all)
+  const float2 *rp32 = nullptr;
+  if (v32_) {
+    if (red) {
+      rp32 = k > 0 ? Rm132_ : nullptr;
+    } else {
+      rp32 = k > 0 ? ring32_ + (int64_t)((k + 2 * ring_k_ - 1) % (2 * ring_k_)) * n_
+                   : nullptr;
+      Yk = nullptr;
+    }
+  }
+  const double2 *rp64 = k > 0 ? Rkm1 : nullptr;
   Timed evp{nullptr, nullptr, red};
   if (timed) {
     NH_HIP(hipEventCreate(&evp.start));
@@ -835,42 +879,46 @@ nh_status LanczosRun::iterate(int k, bool red) {
                          dim3(kThreads), 0, c->stream, c->n_atoms, c->onsite,  \
                          BLK, c->nbr, R_, Q, Y_, bprev, part);                 \
   } while (0)
-#define LZ_SK2(NSP, R, SRC, QUAD, FUSE)                                         \
-  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE, decltype(SRC)>),           \
+// XO, RV, RP: storage type of r, own entries r_k, r_{k-1}
+#define LZ_SK2(NSP, R, SRC, QUAD, FUSE, XO, RV, RP)                             \
+  hipLaunchKernelGGL((k_lz_hv_sk<NSP, R, QUAD, FUSE, decltype(SRC), XO>),       \
                      dim3(gridA), dim3(kSkThreads), 0, c->stream, c->n_cations, \
                      c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,    \
-                     c->nbr, Rk, g32, Q, Yk, bprev, part, alpha_ + k,           \
-                     k > 0 ? Rkm1 : nullptr, bprev2, y32)
-#define LZ_SK1(NSP, R, SRC, QUAD)                                               \
+                     c->nbr, RV, g32, Q, Yk, bprev, part, alpha_ + k, RP,       \
+                     bprev2, y32)
+#define LZ_SK1(NSP, R, SRC, QUAD, XO, RV, RP)                                   \
   do {                                                                          \
-    if (fuse) LZ_SK2(NSP, R, SRC, QUAD, true);                                  \
-    else LZ_SK2(NSP, R, SRC, QUAD, false);                                      \
+    if (fuse) LZ_SK2(NSP, R, SRC, QUAD, true, XO, RV, RP);                      \
+    else LZ_SK2(NSP, R, SRC, QUAD, false, XO, RV, RP);                          \
   } while (0)
-#define LZ_SPL2(R, SRC, FUSE)                                                    \
-  hipLaunchKernelGGL((k_lz_hv_sk_split<R, FUSE, decltype(SRC)>), dim3(gridA),   \
-                     dim3(kSkThreads), 0, c->stream, c->n_cations, c->cations,  \
-                     c->n_anions, c->anions, sg, c->onsite, SRC, c->nbr, Rk,    \
-                     g32, Q, Yk, bprev, part, alpha_ + k,                       \
-                     k > 0 ? Rkm1 : nullptr, bprev2, y32)
-#define LZ_SK0(NSP, R, SRC)                                                     \
+#define LZ_SPL2(R, SRC, FUSE, XO, RV, RP)                                        \
+  hipLaunchKernelGGL((k_lz_hv_sk_split<R, FUSE, decltype(SRC), XO>),            \
+                     dim3(gridA), dim3(kSkThreads), 0, c->stream, c->n_cations, \
+                     c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,    \
+                     c->nbr, RV, g32, Q, Yk, bprev, part, alpha_ + k, RP,       \
+                     bprev2, y32)
+#define LZ_SK0(NSP, R, SRC, XO, RV, RP)                                         \
   do {                                                                          \
     if (NSP == 2 && split) {                                                    \
-      if (fuse) LZ_SPL2(R, SRC, true);                                          \
-      else LZ_SPL2(R, SRC, false);                                              \
-    } else if (use_staged(c)) LZ_SK1(NSP, R, SRC, true);                        \
-    else LZ_SK1(NSP, R, SRC, false);                                            \
+      if (fuse) LZ_SPL2(R, SRC, true, XO, RV, RP);                              \
+      else LZ_SPL2(R, SRC, false, XO, RV, RP);                                  \
+    } else if (use_staged(c)) LZ_SK1(NSP, R, SRC, true, XO, RV, RP);            \
+    else LZ_SK1(NSP, R, SRC, false, XO, RV, RP);                                \
   } while (0)
-#define LZ_SK(R, SRC)                                                           \
+#define LZ_SK(R, SRC, XO, RV, RP)                                               \
   do {                                                                          \
-    if (c->nb == 20) LZ_SK0(2, R, SRC);                                         \
-    else LZ_SK0(1, R, SRC);                                                     \
+    if (c->nb == 20) LZ_SK0(2, R, SRC, XO, RV, RP);                             \
+    else LZ_SK0(1, R, SRC, XO, RV, RP);                                         \
   } while (0)
   if (c->layout == NH_LAYOUT_MATRIX_FREE) {
-    if (c->precision == NH_PREC_FP64) LZ_SK(double, mf_src<double>(c));
-    else LZ_SK(float, mf_src<float>(c));
+    if (c->precision == NH_PREC_FP64) LZ_SK(double, mf_src<double>(c), double2, Rk, rp64);
+    else if (v32_) LZ_SK(float, mf_src<float>(c), float2, g32, rp32);
+    else LZ_SK(float, mf_src<float>(c), double2, Rk, rp64);
   } else if (c->layout == NH_LAYOUT_SK) {
-    if (c->precision == NH_PREC_FP64) LZ_SK(double, (SkSrc<double>{c->sk64, c->bond_of}));
-    else LZ_SK(float, (SkSrc<float>{c->sk32, c->bond_of}));
+    if (c->precision == NH_PREC_FP64)
+      LZ_SK(double, (SkSrc<double>{c->sk64, c->bond_of}), double2, Rk, rp64);
+    else if (v32_) LZ_SK(float, (SkSrc<float>{c->sk32, c->bond_of}), float2, g32, rp32);
+    else LZ_SK(float, (SkSrc<float>{c->sk32, c->bond_of}), double2, Rk, rp64);
   } else if (c->precision == NH_PREC_FP64) {
     if (c->nb == 20) LZ_HV(2, double, c->blocks64);
     else LZ_HV(1, double, c->blocks64);
@@ -892,8 +940,12 @@ nh_status LanczosRun::iterate(int k, bool red) {
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        gridA, 1, part_, alpha_ + k, 0);
   const int nb = reduce_blocks(n_);
-  if (!fuse)
-    hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
+  if (!fuse && v32_)
+    hipLaunchKernelGGL(k_lz_update<float2>, dim3(nb), dim3(kRedThreads), 0, c->stream,
+                       n_, Y_, R32_, k > 0 ? Rm132_ : nullptr, alpha_ + k, bprev,
+                       bprev2, part, y32);
+  else if (!fuse)
+    hipLaunchKernelGGL(k_lz_update<double2>, dim3(nb), dim3(kRedThreads), 0, c->stream,
                        n_, Y_, R_, k > 0 ? Rm1_ : nullptr, alpha_ + k, bprev,
                        bprev2, part, y32);
   if (red)
@@ -901,11 +953,18 @@ nh_status LanczosRun::iterate(int k, bool red) {
                        nb, 2, part_, beta_ + k, 1);
   NH_HIP(hipGetLastError());
   // r_{k+1} is in Y: rotate (r_{k-1}, r_k, y) -> (r_k, r_{k+1}, free)
-  double2 *free_buf = Rm1_;
-  Rm1_ = R_;
-  R_ = Y_;
-  Y_ = free_buf;
-  std::swap(R32_, Y32_);
+  if (v32_) {
+    float2 *free32 = Rm132_;
+    Rm132_ = R32_;
+    R32_ = Y32_;
+    Y32_ = free32;
+  } else {
+    double2 *free_buf = Rm1_;
+    Rm1_ = R_;
+    R_ = Y_;
+    Y_ = free_buf;
+    std::swap(R32_, Y32_);
+  }
   return NH_OK;
 }
 
@@ -1027,9 +1086,12 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   NH_CHECK(o->max_iter > 2 && o->conv_tolerance > 0, NH_ERR_ARG,
            "Lanczos: bad max_iter / tolerance");
   n_ = c->n;
-  NH_TRY(alloc((void **)&R_, n_ * sizeof(double2)));
+  v32_ = lz_fp32_vectors(c);
+  if (!v32_) {
+    NH_TRY(alloc((void **)&R_, n_ * sizeof(double2)));
+    NH_TRY(alloc((void **)&Rm1_, n_ * sizeof(double2)));
+  }
   NH_TRY(alloc((void **)&Y_, n_ * sizeof(double2)));
-  NH_TRY(alloc((void **)&Rm1_, n_ * sizeof(double2)));
   NH_TRY(alloc((void **)&alpha_, (size_t)(o->max_iter + 1) * sizeof(double)));
   NH_TRY(alloc((void **)&beta_, (size_t)(o->max_iter + 1) * sizeof(double)));
   nparts_ = std::max((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock,
@@ -1040,6 +1102,7 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   if (sk_like(c) && c->precision == NH_PREC_FP32) {
     NH_TRY(alloc((void **)&R32_, n_ * sizeof(float2)));
     NH_TRY(alloc((void **)&Y32_, n_ * sizeof(float2)));
+    if (v32_) NH_TRY(alloc((void **)&Rm132_, n_ * sizeof(float2)));
   }
   NH_TRY(upload_start(R32_));
 

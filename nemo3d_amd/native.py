@@ -21,6 +21,7 @@ SET_RING_STEPS = 2
 SET_BLOCK_HV = 3
 SET_SK_ROLES = 4
 SET_SK_LANES = 5
+SET_LZ_VECTORS = 6
 
 _STATUS = {0: "NH_OK", 1: "NH_ERR_ARG", 2: "NH_ERR_HIP", 3: "NH_ERR_STATE",
            4: "NH_ERR_NOMEM", 5: "NH_ERR_NOCONV"}

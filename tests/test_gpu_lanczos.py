@@ -212,10 +212,11 @@ def test_tracemin_on_the_reference_example():
     assert np.abs(np.sort(r["eigenvalues"]) - want).max() < 1e-9
 
 
-def test_mixed_precision_matches_dense():
-    """FP32-product Lanczos (FP64 recurrence) + FP64 refinement: eigenvalues
-    to the FP64 bar, FP64 residuals below tol * hnorm, operator back in
-    FP64 afterwards."""
+@pytest.mark.parametrize("vectors", ["fp64", "fp32"])
+def test_mixed_precision_matches_dense(vectors):
+    """FP32-product Lanczos (FP64 recurrence; its vectors stored in FP64 or
+    FP32) + FP64 refinement: eigenvalues to the FP64 bar, FP64 residuals
+    below tol * hnorm, operator back in FP64 afterwards."""
     from nemo3d_amd.solvers import eigensolve_mixed
     M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=23)
     ref = oracle.assemble(M.arrays())
@@ -223,6 +224,7 @@ def test_mixed_precision_matches_dense():
     sel = np.nonzero((w > -1.0) & (w < 3.0))[0]
     k = sel[np.argmax(np.diff(w[sel]))]
     H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    H.set(native.SET_LZ_VECTORS, 1 if vectors == "fp32" else 0)
     r = eigensolve_mixed(H, num_vb=2, num_cb=2,
                          erange_vb=(0.5 * (w[k - 2] + w[k - 1]), w[k] + 1e-3),
                          erange_cb=(w[k + 1] - 1e-3, 0.5 * (w[k + 2] + w[k + 3])),
@@ -242,17 +244,25 @@ def test_mixed_precision_matches_dense():
     assert H.precision == native.PREC_FP64
 
 
-def test_fp32_lanczos_vectors_at_fp32_level():
+@pytest.mark.parametrize("layout,vectors,lanes", [("sk", "fp64", -1), ("sk", "fp32", 4),
+                                                   ("sk", "fp32", 8), ("mf", "fp32", 4),
+                                                   ("mf", "fp32", 8)])
+def test_fp32_lanczos_vectors_at_fp32_level(layout, vectors, lanes):
     """Lanczos with FP32 products: the eigenvector pass accumulates the FP32
     copies of r (its ring) with 1/beta folded into the coefficients; the
     vectors' FP64 residuals and eigenvalues sit at the FP32 rounding of H·v,
-    far below the gap to the next level, and the FP64 run stays exact."""
+    far below the gap to the next level, and the FP64 run stays exact.
+    With the Lanczos vectors themselves stored in FP32 (the FP64 step's
+    result rounded once) the eigenvector pass must still retrace the first
+    pass exactly, or its Ritz vectors would not be eigenvectors."""
     M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=24)
     ref = oracle.assemble(M.arrays())
     wd, vd = np.linalg.eigh(ref.dense())
     w = distinct(wd)
     wv, wc = gap_windows(w)
-    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    H = Hamiltonian(M, layout=native.LAYOUT_SK if layout == "sk" else native.LAYOUT_MATRIX_FREE)
+    H.set(native.SET_LZ_VECTORS, 1 if vectors == "fp32" else 0)
+    H.set(native.SET_SK_LANES, lanes)
     out = {}
     for prec in (native.PREC_FP32, native.PREC_FP64):
         H.set_precision(prec)
@@ -271,7 +281,8 @@ def test_fp32_lanczos_vectors_at_fp32_level():
             v = v / np.linalg.norm(v)
             assert np.abs(wd - e).min() < bar_e, (prec, e)
             res = np.linalg.norm(ref.hv(v) - e * v)
-            print("precision", prec, "E", e, "residual / ||H||", res / 40.0, flush=True)
+            print(layout, vectors, lanes, "precision", prec, "E", e, "residual / ||H||",
+                  res / 40.0, "iterations", r["iterations"], flush=True)
             assert res < bar_r * 40.0, (prec, e, res)
             # the vector lies in the eigenspace of its level (Kramers pair)
             sel = np.abs(wd - e) < 1e-6
