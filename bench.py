@@ -119,13 +119,13 @@ def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False, fp32_vectors=False):
     and y written once (q is stored only in the eigenvector pass); with
     FP32 products the neighbours' entries come from the FP32 copy of x
     (8 bytes per entry, read once).  fp32_vectors: the Lanczos vectors are
-    FP32, so x is read once as FP32 (own entries and neighbours) and y
-    (FP64) written once."""
+    FP32 and the step is one launch: u_(k-1) and r_(k-1) read once (own
+    entries and neighbours), r_k and u_k written, all FP32."""
     A = M.n_atoms
     nb = int((M.structure.nbr >= 0).sum()) // 2
     mat = A * 18 * 8 + nb * 24 * rec_bytes + A * 4 * 4 * 2 + A * 4
     if fp32_vectors:
-        return mat + M.n * (8 + 16)
+        return mat + M.n * 4 * 8
     return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 
@@ -232,12 +232,12 @@ def mf_lanczos_bytes(M, fp32_copy=False, fp32_vectors=False):
     layout: onsite record (18 doubles), displacement + species (4 doubles),
     the packed ideal bond vectors and the neighbour indices (4 + 4 int32)
     and the list entry per atom, x read once (FP64), y written once, and
-    with FP32 products the FP32 copy of x (read once); fp32_vectors: x read
-    once as FP32, y (FP64) written once."""
+    with FP32 products the FP32 copy of x (read once); fp32_vectors: u_(k-1)
+    and r_(k-1) read, r_k and u_k written, all FP32."""
     A = M.n_atoms
     mat = A * (18 * 8 + 4 * 8 + 4 * 4 * 2 + 4)
     if fp32_vectors:
-        return mat + M.n * (8 + 16)
+        return mat + M.n * 4 * 8
     return mat + 2 * M.n * 16 + (M.n * 8 if fp32_copy else 0)
 
 

@@ -115,11 +115,15 @@ enum { NH_SET_SK_ROLES = 4 };
 enum { NH_SET_SK_LANES = 5 };
 /* NH_SET_LZ_VECTORS: storage of the Lanczos vectors r_k of a Lanczos run
  * with FP32 SK or matrix-free products: 0 FP64 (with an FP32 copy for the
- * products' gathers), 1 FP32 (the three-term step still computed in FP64,
- * its result rounded once; no FP64 vector in either pass), -1 the default
- * (1: measured as fast or faster, with the same number of steps).  Either
- * way the eigenvector pass retraces the eigenvalue pass bit for bit; other
- * layouts and FP64 products always keep FP64 vectors. */
+ * products' gathers), 1 FP32: the step runs in Paige's order, u_k = H q_k
+ * - beta_k q_{k-1} and r_{k+1} = u_k - (alpha_k / beta_k) r_k each
+ * computed in FP64 and rounded once to FP32, and the product of step k
+ * forms r_k from u_{k-1} and r_{k-1} as it reads them, so a step is one
+ * launch (no FP64 vector in either pass; alpha, beta and the sums stay
+ * FP64); -1 the default (1 where the products use the
+ * LDS gather: measured faster, with the same number of steps).  Either
+ * way the eigenvector pass retraces the eigenvalue pass; other layouts
+ * and FP64 products always keep FP64 vectors. */
 enum { NH_SET_LZ_VECTORS = 6 };
 nh_status nh_context_set(nh_context *ctx, int what, int value);
 

@@ -340,16 +340,46 @@ struct SkQuadStage {
   uint4 buf[kSkThreads * kStride];
 };
 
+// FP32 Lanczos vectors: r_k = u_{k-1} - alpha_{k-1} q_{k-1} with q_{k-1} =
+// r_{k-1} / beta_{k-1}, one FP64 fma with a = alpha_{k-1} / beta_{k-1}
+// (lz_defer_coef), rounded once to FP32.  Every kernel that forms r_k (the
+// products that read u_{k-1} and r_{k-1}, the eigenvector pass that writes
+// r_k) evaluates these expressions on the same operands, so they agree bit
+// for bit.  (Measured on a 786,520-atom dot: the same step as an FP32 fma
+// with a rounded to FP32 puts an error along q_{k-1} into every step; the
+// run needed 10,251 steps instead of 7,901, and with another start missed
+// a level.)
+__device__ __forceinline__ double lz_defer_coef(double al, double inv) { return al * inv; }
+__device__ __forceinline__ float lz_defer1(float u, float rp, double a) {
+  return (float)__builtin_fma(-a, (double)rp, (double)u);
+}
+// where the products read r_k from: x holds u_{k-1}, rp r_{k-1}
+struct LzDeferSrc {
+  const float2 *rp;
+  double a;  // lz_defer_coef(alpha_{k-1}, 1 / beta_{k-1})
+};
+__device__ __forceinline__ uint4 lz_defer_chunk(uint4 u, uint4 rp, const LzDeferSrc &d) {
+  uint4 o;
+  o.x = __float_as_uint(lz_defer1(__uint_as_float(u.x), __uint_as_float(rp.x), d.a));
+  o.y = __float_as_uint(lz_defer1(__uint_as_float(u.y), __uint_as_float(rp.y), d.a));
+  o.z = __float_as_uint(lz_defer1(__uint_as_float(u.z), __uint_as_float(rp.z), d.a));
+  o.w = __float_as_uint(lz_defer1(__uint_as_float(u.w), __uint_as_float(rp.w), d.a));
+  return o;
+}
+
 // The NSP * 10 entries of an item start at entry `off` of the neighbour's
 // `stride` entries (stride = NSP * 10, off = 0: the whole atom; stride =
-// 20, off = 10 s: spin block s of a 20-band atom).
-template <int NSP, typename R, typename XT>
+// 20, off = 10 s: spin block s of a 20-band atom).  DEFER (XT = float2):
+// the staged entries are r_k formed from x = u_{k-1} and d.rp = r_{k-1}.
+template <int NSP, typename R, typename XT, bool DEFER = false>
 __device__ __forceinline__ void sk_quad_gather(int j, int slot,
                                                const XT *__restrict__ x,
                                                SkQuadStage<NSP, XT> &L,
                                                R (&xr)[NSP][10],
                                                R (&xi)[NSP][10],
-                                               int stride = NSP * 10, int off = 0) {
+                                               int stride = NSP * 10, int off = 0,
+                                               LzDeferSrc d = {nullptr, 0.0}) {
+  static_assert(!DEFER || sizeof(XT) == 8, "deferred r_k needs FP32 vectors");
   using S = SkQuadStage<NSP, XT>;
   constexpr int NB = NSP * 10;
   constexpr int C = S::kChunks;
@@ -359,10 +389,20 @@ __device__ __forceinline__ void sk_quad_gather(int j, int slot,
     const int jn = __shfl(j, (t & 63 & ~3) | n, 64);
     if (jn >= 0) {
       const uint4 *src = reinterpret_cast<const uint4 *>(x + (int64_t)jn * stride + off);
+      if constexpr (DEFER) {
+        const uint4 *srp =
+            reinterpret_cast<const uint4 *>(d.rp + (int64_t)jn * stride + off);
 #pragma unroll
-      for (int c0 = 0; c0 < C; c0 += 4) {
-        const int c = c0 + slot;
-        if (c < C) L.buf[S::pos(q0 + n, c)] = src[c];
+        for (int c0 = 0; c0 < C; c0 += 4) {
+          const int c = c0 + slot;
+          if (c < C) L.buf[S::pos(q0 + n, c)] = lz_defer_chunk(src[c], srp[c], d);
+        }
+      } else {
+#pragma unroll
+        for (int c0 = 0; c0 < C; c0 += 4) {
+          const int c = c0 + slot;
+          if (c < C) L.buf[S::pos(q0 + n, c)] = src[c];
+        }
       }
     }
   }
@@ -462,14 +502,17 @@ constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
 // from the caches.  Callers branch on the role with a wave-uniform
 // (scalar) condition, so every wave runs exactly one instantiation and
 // reaches the same number of barriers whichever role it holds.
-template <int NSP, typename R, bool CATION, typename XT, bool QUAD, typename SRC>
+template <int NSP, typename R, bool CATION, typename XT, bool QUAD, bool DEFER = false,
+          typename SRC>
 __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
                                          const SRC &src,
                                          const int32_t *__restrict__ nbr,
                                          const XT *__restrict__ xc,
                                          double (&own_re)[(NSP * 10 + 3) / 4],
                                          double (&own_im)[(NSP * 10 + 3) / 4],
-                                         SkQuadStage<NSP, XT> *L = nullptr) {
+                                         SkQuadStage<NSP, XT> *L = nullptr,
+                                         LzDeferSrc d = {nullptr, 0.0}) {
+  static_assert(!DEFER || QUAD, "deferred r_k needs the LDS gather");
   R br[NSP][10], bi[NSP][10];
 #pragma unroll
   for (int s = 0; s < NSP; ++s)
@@ -480,7 +523,7 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
   if constexpr (QUAD) {
     R rv[24];
     if (SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
-    sk_quad_gather<NSP, R, XT>(j, slot, xc, *L, xr, xi);
+    sk_quad_gather<NSP, R, XT, DEFER>(j, slot, xc, *L, xr, xi, NSP * 10, 0, d);
     if (!SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
     if (j >= 0)
       BondApply<NSP, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3,
@@ -523,14 +566,15 @@ struct SkSplit {
   static constexpr int kOwn = 3;  // rows per lane
 };
 
-template <typename R, bool CATION, typename XT, typename SRC>
+template <typename R, bool CATION, typename XT, bool DEFER = false, typename SRC>
 __device__ __forceinline__ void sk_bonds_split(int atom, int slot, int spin,
                                                bool active, const SRC &src,
                                                const int32_t *__restrict__ nbr,
                                                const XT *__restrict__ xc,
                                                double (&own_re)[SkSplit::kOwn],
                                                double (&own_im)[SkSplit::kOwn],
-                                               SkQuadStage<1, XT> &L) {
+                                               SkQuadStage<1, XT> &L,
+                                               LzDeferSrc d = {nullptr, 0.0}) {
   R br[1][10], bi[1][10];
 #pragma unroll
   for (int c = 0; c < 10; ++c) br[0][c] = bi[0][c] = R(0);
@@ -538,7 +582,7 @@ __device__ __forceinline__ void sk_bonds_split(int atom, int slot, int spin,
   R xr[1][10], xi[1][10];
   R rv[24];
   if (SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
-  sk_quad_gather<1, R, XT>(j, slot, xc, L, xr, xi, 20, 10 * spin);
+  sk_quad_gather<1, R, XT, DEFER>(j, slot, xc, L, xr, xi, 20, 10 * spin, d);
   if (!SRC::kEarly && j >= 0) src.template fetch<CATION>(atom, slot, j, rv);
   if (j >= 0)
     BondApply<1, R, CATION>::template run<0>(rv[0], rv[1], rv[2], rv + 3, xr,

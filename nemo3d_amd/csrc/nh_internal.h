@@ -159,12 +159,14 @@ inline bool sk_split_lanes(const nh_context *c) {
     return c->layout == NH_LAYOUT_MATRIX_FREE && c->precision == NH_PREC_FP32;
   return c->sk_lanes == 8;
 }
-// Lanczos vectors stored in FP32 (FP32 SK or matrix-free products only).
-// Default on (measured on a 786,520-atom dot, FP32 SK products: the same
-// 7,901 steps to the same eigenvalues, fused H*v 0.208 / 0.228 ms in the
-// two passes against 0.229 / 0.301 ms, FP32 stage 4.73 s against 6.23 s)
+// Lanczos vectors stored in FP32 (FP32 SK or matrix-free products with the
+// LDS gather only).  Default on (measured on a 786,520-atom dot, FP32 SK
+// products: the same 7,901 steps to the same eigenvalues, fused H*v 0.208
+// / 0.228 ms in the two passes against 0.229 / 0.301 ms, FP32 stage 4.73 s
+// against 6.23 s, with a separate update kernel)
 inline bool lz_fp32_vectors(const nh_context *c) {
-  return sk_like(c) && c->precision == NH_PREC_FP32 && c->lz_vectors != 0;
+  return sk_like(c) && c->precision == NH_PREC_FP32 && use_staged(c) &&
+         c->lz_vectors != 0;
 }
 }  // namespace nh
 

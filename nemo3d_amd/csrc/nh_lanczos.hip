@@ -6,6 +6,10 @@
 //   k_lz_hv     q = r / beta_{k-1} (written into a ring slot), y = H q, and
 //               per-workgroup partials of alpha = Re<q, y>;
 //   k_lz_update r = y - alpha q - beta_{k-1} q_prev with partials of |r|^2.
+// With FP32 products and FP32 Lanczos vectors (SK and matrix-free layouts)
+// it is one kernel and one reduction: the product forms r_k from u_{k-1}
+// and r_{k-1} as it reads them, computes u_k = H q_k - beta_k q_{k-1} and
+// the partials of alpha_k, |u_k|^2 and |q_k|^2 (k_lz_hv_sk, Paige's order).
 // alpha and beta stay in device arrays; the host copies the tridiagonal
 // every ConvCheckSkipRate iterations and analyses it (Sturm bisection in
 // the Erange windows, convergence by the trailing components of the
@@ -142,11 +146,46 @@ __device__ __forceinline__ double2 lz_three_term(double2 y, double2 q,
                          rp ? widen2(*rp) : make_double2(0.0, 0.0), al, be, invp);
 }
 
+// FP32 Lanczos vectors (Paige's order of the step): u_k = H q_k - beta_k
+// q_{k-1} computed in FP64 and rounded to FP32, alpha_k = <q_k, u_k>,
+// r_{k+1} = u_k - alpha_k q_k (lz_defer1).  y is kept opaque so that every
+// kernel rounds u_k from the same two operations.
+__device__ __forceinline__ float2 lz_u32(double2 y, double2 rp, double be, double invp) {
+  asm volatile("" : "+v"(y.x), "+v"(y.y));
+  return make_float2((float)__builtin_fma(-be, rp.x * invp, y.x),
+                     (float)__builtin_fma(-be, rp.y * invp, y.y));
+}
+// per-workgroup partials of the FP32-vector eigenvalue pass: alpha_k,
+// |u_k|^2, |q_k|^2 (beta_{k+1}^2 = |u|^2 - 2 alpha^2 + alpha^2 |q|^2)
+struct LzPart3 {
+  double a = 0.0, u = 0.0, q = 0.0;
+  __device__ __forceinline__ void add(double2 qv, float2 uv) {
+    const double ux = uv.x, uy = uv.y;
+    a += qv.x * ux + qv.y * uy;
+    u += ux * ux + uy * uy;
+    q += qv.x * qv.x + qv.y * qv.y;
+  }
+  template <bool QUAD, typename S>
+  __device__ __forceinline__ void store(double *part, S *L) {
+    LzBlockSum<QUAD>::run(a, u, L);
+    double z = 0.0;
+    LzBlockSum<QUAD>::run(q, z, L);
+    if (threadIdx.x == 0) {
+      part[3 * blockIdx.x] = a;
+      part[3 * blockIdx.x + 1] = u;
+      part[3 * blockIdx.x + 2] = q;
+    }
+  }
+};
+
 // XO: storage of the Lanczos vectors r.  double2: FP64 r (own entries,
-// r_prev) with an FP32 copy for the gathers of FP32 products; float2 (FP32
-// products only): the r vectors themselves are FP32 (Rv == Rv32), the
-// eigenvector pass writes the rounded r_{k+1} to Y32 only, and the step's
-// arithmetic stays FP64.
+// r_prev) with an FP32 copy for the gathers of FP32 products.  float2
+// (FP32 products, LDS gather): the vectors are FP32 and the step runs in
+// Paige's order (lz_u32, lz_defer1).  Eigenvalue pass: Rv = u_{k-1}, Rp =
+// r_{k-1}; r_k is formed on the fly (gather and own entries) and written
+// to Rout, u_k to Y32, the partials of LzPart3 to part (no update
+// kernel).  Eigenvector pass (FUSE): Rv = Rv32 = r_k, Rp = r_{k-1}, the
+// rounded r_{k+1} to Y32.  The step's arithmetic stays FP64.
 template <int NSP, typename R, bool QUAD, bool FUSE, typename SRC, typename XO = double2>
 __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk(
     int nc, const int32_t *__restrict__ cations, int na,
@@ -156,10 +195,13 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
     double2 *__restrict__ Q, double2 *__restrict__ Y,
     const double *__restrict__ beta_prev, double *__restrict__ part,
     const double *__restrict__ alpha, const XO *__restrict__ Rp,
-    const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
+    const double *__restrict__ beta_prev2, float2 *__restrict__ Y32,
+    float2 *__restrict__ Rout, const double *__restrict__ alpha_prev) {
   constexpr int NB = NSP * 10;
   constexpr bool V32 = std::is_same<XO, float2>::value;
-  static_assert(!V32 || sizeof(R) == 4, "FP32 Lanczos vectors need FP32 products");
+  constexpr bool DEFER = V32 && !FUSE;
+  static_assert(!V32 || (sizeof(R) == 4 && QUAD),
+                "FP32 Lanczos vectors need FP32 products and the LDS gather");
   constexpr int NO = (NB + 3) / 4;
   const SkLane ln = sk_lane_of(grid, blockIdx.x, threadIdx.x);
   // the role is uniform across the wavefront; read as a scalar, the role
@@ -182,18 +224,25 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
   const XT *xg;
   if constexpr (sizeof(R) == 4) xg = Rv32;
   else xg = Rv;
+  // DEFER: r_k = lz_defer1(u_{k-1}, r_{k-1}, alpha_{k-1} / beta_{k-1});
+  // FUSE: r_{k+1} = lz_defer1(u_k, r_k, alpha_k / beta_k)
+  const double alp = DEFER && alpha_prev ? alpha_prev[0] : 0.0;
+  const double ap = lz_defer_coef(alp, invp), af = lz_defer_coef(al, inv);
+  LzDeferSrc dsrc{nullptr, ap};
+  if constexpr (DEFER) dsrc.rp = Rp;
   SkQuadStage<NSP, XT> *L = nullptr;
   if constexpr (QUAD) {
     __shared__ SkQuadStage<NSP, XT> stage;
     L = &stage;
   }
   if (cat)
-    sk_bonds<NSP, R, true, XT, QUAD>(atom, slot, active, src, nbr, xg, own_re,
-                                     own_im, L);
+    sk_bonds<NSP, R, true, XT, QUAD, DEFER>(atom, slot, active, src, nbr, xg, own_re,
+                                            own_im, L, dsrc);
   else
-    sk_bonds<NSP, R, false, XT, QUAD>(atom, slot, active, src, nbr, xg, own_re,
-                                      own_im, L);
+    sk_bonds<NSP, R, false, XT, QUAD, DEFER>(atom, slot, active, src, nbr, xg, own_re,
+                                             own_im, L, dsrc);
   double local = 0.0;
+  LzPart3 p3;
   if constexpr (QUAD) {
     static_assert(sizeof(SkOwnStage<NSP>) <= sizeof(SkQuadStage<NSP, XT>),
                   "epilogue staging must fit the gather staging");
@@ -209,11 +258,26 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
     double2 rpv[NO];
 #pragma unroll
     for (int i = 0; i < NO; ++i) rpv[i] = make_double2(0.0, 0.0);
-    if constexpr (FUSE) {
+    if constexpr (FUSE || DEFER) {
       if (active && Rp) {
 #pragma unroll
         for (int i = 0; i < NO; ++i)
           if (4 * i + slot < NB) rpv[i] = widen2(Rp[(int64_t)atom * NB + 4 * i + slot]);
+      }
+    }
+    if constexpr (DEFER) {
+      // own entries of r_k (the lane's rows 4 i + slot, as loaded) from
+      // u_{k-1} and r_{k-1}; written out for the next step
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < NO; ++i) {
+          const int r = 4 * i + slot;
+          if (r >= NB) break;
+          const float2 rk = make_float2(lz_defer1((float)own.x[i].x, (float)rpv[i].x, ap),
+                                        lz_defer1((float)own.x[i].y, (float)rpv[i].y, ap));
+          own.x[i] = widen2(rk);
+          if (Rout) Rout[(int64_t)atom * NB + r] = rk;
+        }
       }
     }
     __syncthreads();  // every lane has read its staged neighbour entries
@@ -230,13 +294,24 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
         const double2 q = make_double2(own.x[i].x * inv, own.x[i].y * inv);
         const int64_t e = (int64_t)atom * NB + r;
         double2 y = make_double2(re * inv, im * inv);
-        if constexpr (FUSE) {
-          y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
-          if (V32 || Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+        if constexpr (V32) {
+          const float2 u = lz_u32(y, rpv[i], be, invp);
+          if constexpr (FUSE) {
+            Y32[e] = make_float2(lz_defer1(u.x, (float)own.x[i].x, af),
+                                 lz_defer1(u.y, (float)own.x[i].y, af));
+          } else {
+            Y32[e] = u;
+            p3.add(q, u);
+          }
+        } else {
+          if constexpr (FUSE) {
+            y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
+            if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+          }
+          if (Q) Q[e] = q;
+          Y[e] = y;
+          local += q.x * y.x + q.y * y.y;
         }
-        if (Q) Q[e] = q;
-        if constexpr (!(V32 && FUSE)) Y[e] = y;
-        local += q.x * y.x + q.y * y.y;
       }
     }
   } else if (active) {
@@ -254,17 +329,21 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
       double2 y = make_double2(re * inv, im * inv);
       if constexpr (FUSE) {
         y = lz_three_term(y, q, Rp ? Rp + e : nullptr, al, be, invp);
-        if (V32 || Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+        if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
       }
       if (Q) Q[e] = q;
-      if constexpr (!(V32 && FUSE)) Y[e] = y;
+      Y[e] = y;
       local += q.x * y.x + q.y * y.y;
     }
   }
   if (part) {
-    double z = 0.0;
-    LzBlockSum<QUAD>::run(local, z, L);  // QUAD: reuses the staging LDS
-    if (threadIdx.x == 0) part[blockIdx.x] = local;
+    if constexpr (DEFER) {
+      p3.store<QUAD>(part, L);
+    } else {
+      double z = 0.0;
+      LzBlockSum<QUAD>::run(local, z, L);  // QUAD: reuses the staging LDS
+      if (threadIdx.x == 0) part[blockIdx.x] = local;
+    }
   }
 }
 
@@ -282,9 +361,11 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
     double2 *__restrict__ Q, double2 *__restrict__ Y,
     const double *__restrict__ beta_prev, double *__restrict__ part,
     const double *__restrict__ alpha, const XO *__restrict__ Rp,
-    const double *__restrict__ beta_prev2, float2 *__restrict__ Y32) {
+    const double *__restrict__ beta_prev2, float2 *__restrict__ Y32,
+    float2 *__restrict__ Rout, const double *__restrict__ alpha_prev) {
   constexpr int NSP = 2, NB = 20;
   constexpr bool V32 = std::is_same<XO, float2>::value;
+  constexpr bool DEFER = V32 && !FUSE;
   static_assert(!V32 || sizeof(R) == 4, "FP32 Lanczos vectors need FP32 products");
   constexpr int kLanes = SkSplit::kLanes, kAtoms = SkSplit::kAtoms;
   const SkLane ln = sk_lane_of<kLanes>(grid, blockIdx.x, threadIdx.x);
@@ -303,13 +384,17 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
   const XT *xg;
   if constexpr (sizeof(R) == 4) xg = Rv32;
   else xg = Rv;
+  const double alp = DEFER && alpha_prev ? alpha_prev[0] : 0.0;
+  const double ap = lz_defer_coef(alp, invp), af = lz_defer_coef(al, inv);
+  LzDeferSrc dsrc{nullptr, ap};
+  if constexpr (DEFER) dsrc.rp = Rp;
   __shared__ SkQuadStage<1, XT> stage;
   if (cat)
-    sk_bonds_split<R, true, XT>(atom, slot, spin, active, src, nbr, xg, own_re,
-                                own_im, stage);
+    sk_bonds_split<R, true, XT, DEFER>(atom, slot, spin, active, src, nbr, xg, own_re,
+                                       own_im, stage, dsrc);
   else
-    sk_bonds_split<R, false, XT>(atom, slot, spin, active, src, nbr, xg, own_re,
-                                 own_im, stage);
+    sk_bonds_split<R, false, XT, DEFER>(atom, slot, spin, active, src, nbr, xg, own_re,
+                                        own_im, stage, dsrc);
   static_assert(sizeof(SkOwnStage<NSP, kAtoms>) <= sizeof(SkQuadStage<1, XT>),
                 "epilogue staging must fit the gather staging");
   auto &S = *reinterpret_cast<SkOwnStage<NSP, kAtoms> *>(stage.buf);
@@ -321,7 +406,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
   double2 rpv[SkSplit::kOwn];
 #pragma unroll
   for (int i = 0; i < SkSplit::kOwn; ++i) rpv[i] = make_double2(0.0, 0.0);
-  if constexpr (FUSE) {
+  if constexpr (FUSE || DEFER) {
     if (active && Rp) {
 #pragma unroll
       for (int i = 0; i < SkSplit::kOwn; ++i)
@@ -329,10 +414,27 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
           rpv[i] = widen2(Rp[(int64_t)atom * NB + spin * 10 + 4 * i + slot]);
     }
   }
+  if constexpr (DEFER) {
+    // own entries of r_k (entries 8 i + lane, as loaded) from u_{k-1} and
+    // r_{k-1}; written out for the next step
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < (NB + kLanes - 1) / kLanes; ++i) {
+        const int r = kLanes * i + lane;
+        if (r >= NB) break;
+        const float2 rq = Rp[(int64_t)atom * NB + r];
+        const float2 rk = make_float2(lz_defer1((float)own.x[i].x, rq.x, ap),
+                                      lz_defer1((float)own.x[i].y, rq.y, ap));
+        own.x[i] = widen2(rk);
+        if (Rout) Rout[(int64_t)atom * NB + r] = rk;
+      }
+    }
+  }
   __syncthreads();  // every lane has read its staged neighbour entries
   if (active) sk_store_own<NSP, kLanes, kAtoms>(la, lane, own, S);
   __syncthreads();
   double local = 0.0;
+  LzPart3 p3;
   if (active) {
 #pragma unroll
     for (int i = 0; i < SkSplit::kOwn; ++i) {
@@ -346,19 +448,34 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
       const double2 q = make_double2(xo.x * inv, xo.y * inv);
       const int64_t e = (int64_t)atom * NB + r;
       double2 y = make_double2(re * inv, im * inv);
-      if constexpr (FUSE) {
-        y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
-        if (V32 || Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+      if constexpr (V32) {
+        const float2 u = lz_u32(y, rpv[i], be, invp);
+        if constexpr (FUSE) {
+          Y32[e] = make_float2(lz_defer1(u.x, (float)xo.x, af),
+                               lz_defer1(u.y, (float)xo.y, af));
+        } else {
+          Y32[e] = u;
+          p3.add(q, u);
+        }
+      } else {
+        if constexpr (FUSE) {
+          y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
+          if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
+        }
+        if (Q) Q[e] = q;
+        Y[e] = y;
+        local += q.x * y.x + q.y * y.y;
       }
-      if (Q) Q[e] = q;
-      if constexpr (!(V32 && FUSE)) Y[e] = y;
-      local += q.x * y.x + q.y * y.y;
     }
   }
   if (part) {
-    double z = 0.0;
-    LzBlockSum<true>::run(local, z, &stage);
-    if (threadIdx.x == 0) part[blockIdx.x] = local;
+    if constexpr (DEFER) {
+      p3.store<true>(part, &stage);
+    } else {
+      double z = 0.0;
+      LzBlockSum<true>::run(local, z, &stage);
+      if (threadIdx.x == 0) part[blockIdx.x] = local;
+    }
   }
 }
 
@@ -368,16 +485,11 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
 // q was stored), partial sums of |y|^2.  Rp may be null (first step), and
 // beta_prev / beta_prev2 null mean 1 (the start vector is normalised).
 // Y32 (optional): FP32 copy of the new r for the next FP32 product.
-// XO = float2: r_k, r_prev are FP32 and the rounded r_{k+1} goes to Y32
-// only (Y keeps H q); |r|^2 is summed over the rounded entries, so that
-// r_{k+1} / beta_k has unit norm.
-template <typename XO>
 __global__ __launch_bounds__(kRedThreads) void k_lz_update(
-    int64_t n, double2 *__restrict__ Y, const XO *__restrict__ Rk,
-    const XO *__restrict__ Rp, const double *__restrict__ alpha,
+    int64_t n, double2 *__restrict__ Y, const double2 *__restrict__ Rk,
+    const double2 *__restrict__ Rp, const double *__restrict__ alpha,
     const double *__restrict__ beta_prev, const double *__restrict__ beta_prev2,
     double *__restrict__ part, float2 *__restrict__ Y32) {
-  constexpr bool V32 = std::is_same<XO, float2>::value;
   const double al = alpha[0];
   const double be = beta_prev ? beta_prev[0] : 0.0;
   const double inv = beta_prev ? 1.0 / beta_prev[0] : 1.0;
@@ -385,18 +497,12 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_update(
   double s = 0.0, z = 0.0;
   for (int64_t k = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * kRedThreads) {
-    const double2 r = widen2(Rk[k]);
+    const double2 r = Rk[k];
     const double2 q = make_double2(r.x * inv, r.y * inv);
     const double2 y = lz_three_term(Y[k], q, Rp ? Rp + k : nullptr, al, be, invp);
-    if constexpr (V32) {
-      const float2 f = make_float2((float)y.x, (float)y.y);
-      Y32[k] = f;
-      s += (double)f.x * f.x + (double)f.y * f.y;
-    } else {
-      Y[k] = y;
-      if (Y32) Y32[k] = make_float2((float)y.x, (float)y.y);
-      s += y.x * y.x + y.y * y.y;
-    }
+    Y[k] = y;
+    if (Y32) Y32[k] = make_float2((float)y.x, (float)y.y);
+    s += y.x * y.x + y.y * y.y;
   }
   if (part) {
     block_sum2(s, z);
@@ -420,6 +526,48 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_scalar(
   double a = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7])), b = 0.0;
   block_sum2(a, b);
   if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(a) : a;
+}
+
+// FP32 Lanczos vectors: alpha_k and beta_{k+1} from the LzPart3 partials
+// of nparts workgroups (fixed summation order):
+// beta_{k+1}^2 = |u_k - alpha_k q_k|^2 = |u|^2 - 2 alpha^2 + alpha^2 |q|^2
+__global__ __launch_bounds__(kRedThreads) void k_lz_scalars3(
+    int nparts, const double *__restrict__ part, double *__restrict__ alpha,
+    double *__restrict__ beta) {
+  // four independent partial sums per component and thread (the loads of
+  // a trip in flight together), combined in a fixed order
+  double p[4][3] = {};
+  int k = threadIdx.x;
+  for (; k + 3 * kRedThreads < nparts; k += 4 * kRedThreads) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) p[u][c] += part[3 * (k + u * kRedThreads) + c];
+  }
+  for (int u = 0; k < nparts; k += kRedThreads, ++u)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[u & 3][c] += part[3 * k + c];
+  double s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[c] = (p[0][c] + p[1][c]) + (p[2][c] + p[3][c]);
+  __shared__ double w[3][kRedThreads / 64];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s[c] += __shfl_down(s[c], off, 64);
+    if ((threadIdx.x & 63) == 0) w[c][threadIdx.x >> 6] = s[c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 0; k < kRedThreads / 64; ++k) t[c] += w[c][k];
+    const double a = t[0];
+    alpha[0] = a;
+    beta[0] = sqrt(fmax(t[1] - 2.0 * a * a + a * a * t[2], 0.0));
+  }
 }
 
 // V[e] += sum_k Z[e][k] Ring[k] for e < ne, k < K <= kRingMax (ring of
@@ -664,11 +812,11 @@ class LanczosRun {
   double2 *R_ = nullptr, *Y_ = nullptr, *Rm1_ = nullptr, *ring_ = nullptr;
   // FP32 copies of r for the FP32 SK products (null otherwise)
   float2 *R32_ = nullptr, *Y32_ = nullptr;
-  // FP32 Lanczos vectors (lz_fp32_vectors): r_k, r_{k-1}, r_{k+1} of the
-  // eigenvalue pass are R32_, Rm132_, Y32_; Y_ receives H q; R_ and Rm1_
-  // are not allocated
+  // FP32 Lanczos vectors (lz_fp32_vectors): the eigenvalue pass keeps
+  // r_{k-1} in R32_ and u_{k-1} in U32_ and writes r_k, u_k to Y32_, UN32_
+  // (step k = one launch plus k_lz_scalars3); R_ and Rm1_ are not allocated
   bool v32_ = false;
-  float2 *Rm132_ = nullptr;
+  float2 *U32_ = nullptr, *UN32_ = nullptr;
   // ring length of the eigenvector pass (<= kRingMax)
   int ring_k_ = kRingMax;
   // eigenvector pass with FP32 products: 2 K FP32 copies of r, slot
@@ -848,16 +996,27 @@ nh_status LanczosRun::iterate(int k, bool red) {
   const bool timed = (k % kTimeEvery) == 0;
   // the eigenvector pass knows alpha_k: the SK kernel applies the update
   const bool fuse = !red && sk_like(c);
-  // FP32 Lanczos vectors: own entries and gathers from r_k (FP32), r_{k-1}
-  // from the previous FP32 slot; the eigenvector pass writes only the FP32
-  // r_{k+1} (no FP64 vector at all)
+  // FP32 Lanczos vectors.  Eigenvalue pass: the product reads u_{k-1}
+  // (own entries and gathers) and r_{k-1}, forms r_k, writes r_k and u_k
+  // (step 0: u_{-1} = r_{-1} = r_0 with alpha_{-1} = beta_0 = 0, which
+  // leaves r_0 as it is; nothing is written back for it).  Eigenvector
+  // pass: own entries and gathers from ring slot k, r_{k-1} from slot k - 1
+  // (slot 0 itself at step 0, the eigenvalue pass's operands), r_{k+1} to
+  // slot k + 1; no FP64 vector at all.
   const float2 *rp32 = nullptr;
+  float2 *rout = nullptr;
+  const double *aprev = nullptr;
   if (v32_) {
     if (red) {
-      rp32 = k > 0 ? Rm132_ : nullptr;
+      g32 = k == 0 ? R32_ : U32_;
+      rp32 = R32_;
+      rout = k == 0 ? nullptr : Y32_;
+      y32 = UN32_;
+      aprev = k > 0 ? alpha_ + (k - 1) : nullptr;
+      Yk = nullptr;
+      part = part_;
     } else {
-      rp32 = k > 0 ? ring32_ + (int64_t)((k + 2 * ring_k_ - 1) % (2 * ring_k_)) * n_
-                   : nullptr;
+      rp32 = ring32_ + (int64_t)((k + 2 * ring_k_ - (k > 0 ? 1 : 0)) % (2 * ring_k_)) * n_;
       Yk = nullptr;
     }
   }
@@ -885,7 +1044,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
                      dim3(gridA), dim3(kSkThreads), 0, c->stream, c->n_cations, \
                      c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,    \
                      c->nbr, RV, g32, Q, Yk, bprev, part, alpha_ + k, RP,       \
-                     bprev2, y32)
+                     bprev2, y32, rout, aprev)
 #define LZ_SK1(NSP, R, SRC, QUAD, XO, RV, RP)                                   \
   do {                                                                          \
     if (fuse) LZ_SK2(NSP, R, SRC, QUAD, true, XO, RV, RP);                      \
@@ -896,7 +1055,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
                      dim3(gridA), dim3(kSkThreads), 0, c->stream, c->n_cations, \
                      c->cations, c->n_anions, c->anions, sg, c->onsite, SRC,    \
                      c->nbr, RV, g32, Q, Yk, bprev, part, alpha_ + k, RP,       \
-                     bprev2, y32)
+                     bprev2, y32, rout, aprev)
 #define LZ_SK0(NSP, R, SRC, XO, RV, RP)                                         \
   do {                                                                          \
     if (NSP == 2 && split) {                                                    \
@@ -910,14 +1069,27 @@ nh_status LanczosRun::iterate(int k, bool red) {
     if (c->nb == 20) LZ_SK0(2, R, SRC, XO, RV, RP);                             \
     else LZ_SK0(1, R, SRC, XO, RV, RP);                                         \
   } while (0)
+// FP32 vectors (LDS gather only)
+#define LZ_SKV0(NSP, SRC)                                                       \
+  do {                                                                          \
+    if (NSP == 2 && split) {                                                    \
+      if (fuse) LZ_SPL2(float, SRC, true, float2, g32, rp32);                   \
+      else LZ_SPL2(float, SRC, false, float2, g32, rp32);                       \
+    } else LZ_SK1(NSP, float, SRC, true, float2, g32, rp32);                    \
+  } while (0)
+#define LZ_SKV(SRC)                                                             \
+  do {                                                                          \
+    if (c->nb == 20) LZ_SKV0(2, SRC);                                           \
+    else LZ_SKV0(1, SRC);                                                       \
+  } while (0)
   if (c->layout == NH_LAYOUT_MATRIX_FREE) {
     if (c->precision == NH_PREC_FP64) LZ_SK(double, mf_src<double>(c), double2, Rk, rp64);
-    else if (v32_) LZ_SK(float, mf_src<float>(c), float2, g32, rp32);
+    else if (v32_) LZ_SKV(mf_src<float>(c));
     else LZ_SK(float, mf_src<float>(c), double2, Rk, rp64);
   } else if (c->layout == NH_LAYOUT_SK) {
     if (c->precision == NH_PREC_FP64)
       LZ_SK(double, (SkSrc<double>{c->sk64, c->bond_of}), double2, Rk, rp64);
-    else if (v32_) LZ_SK(float, (SkSrc<float>{c->sk32, c->bond_of}), float2, g32, rp32);
+    else if (v32_) LZ_SKV((SkSrc<float>{c->sk32, c->bond_of}));
     else LZ_SK(float, (SkSrc<float>{c->sk32, c->bond_of}), double2, Rk, rp64);
   } else if (c->precision == NH_PREC_FP64) {
     if (c->nb == 20) LZ_HV(2, double, c->blocks64);
@@ -932,20 +1104,28 @@ nh_status LanczosRun::iterate(int k, bool red) {
 #undef LZ_SK1
 #undef LZ_SK2
 #undef LZ_SPL2
+#undef LZ_SKV
+#undef LZ_SKV0
   if (timed) {
     NH_HIP(hipEventRecord(evp.stop, c->stream));
     ev_.push_back(evp);
+  }
+  if (v32_ && red) {
+    // alpha_k, beta_{k+1} from the product's partials; rotate
+    // (r_{k-1}, u_{k-1}) <- (r_k, u_k) (step 0: r_0 stays)
+    hipLaunchKernelGGL(k_lz_scalars3, dim3(1), dim3(kRedThreads), 0, c->stream,
+                       gridA, part_, alpha_ + k, beta_ + k);
+    NH_HIP(hipGetLastError());
+    if (k > 0) std::swap(R32_, Y32_);
+    std::swap(U32_, UN32_);
+    return NH_OK;
   }
   if (red)
     hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(kRedThreads), 0, c->stream,
                        gridA, 1, part_, alpha_ + k, 0);
   const int nb = reduce_blocks(n_);
-  if (!fuse && v32_)
-    hipLaunchKernelGGL(k_lz_update<float2>, dim3(nb), dim3(kRedThreads), 0, c->stream,
-                       n_, Y_, R32_, k > 0 ? Rm132_ : nullptr, alpha_ + k, bprev,
-                       bprev2, part, y32);
-  else if (!fuse)
-    hipLaunchKernelGGL(k_lz_update<double2>, dim3(nb), dim3(kRedThreads), 0, c->stream,
+  if (!fuse)
+    hipLaunchKernelGGL(k_lz_update, dim3(nb), dim3(kRedThreads), 0, c->stream,
                        n_, Y_, R_, k > 0 ? Rm1_ : nullptr, alpha_ + k, bprev,
                        bprev2, part, y32);
   if (red)
@@ -954,10 +1134,7 @@ nh_status LanczosRun::iterate(int k, bool red) {
   NH_HIP(hipGetLastError());
   // r_{k+1} is in Y: rotate (r_{k-1}, r_k, y) -> (r_k, r_{k+1}, free)
   if (v32_) {
-    float2 *free32 = Rm132_;
-    Rm132_ = R32_;
-    R32_ = Y32_;
-    Y32_ = free32;
+    // eigenvector pass: the ring slots rotate by index
   } else {
     double2 *free_buf = Rm1_;
     Rm1_ = R_;
@@ -1096,13 +1273,20 @@ nh_status LanczosRun::run(const nh_lanczos_opts *o, nh_eig_result *res) {
   NH_TRY(alloc((void **)&beta_, (size_t)(o->max_iter + 1) * sizeof(double)));
   nparts_ = std::max((c->n_atoms + kAtomsPerBlock - 1) / kAtomsPerBlock,
                      2 * kRedMaxBlocks) + 64;
-  NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 2 * sizeof(double)));
+  NH_TRY(alloc((void **)&part_, (size_t)nparts_ * 3 * sizeof(double)));
 
   NH_TRY(build_start(o));
   if (sk_like(c) && c->precision == NH_PREC_FP32) {
     NH_TRY(alloc((void **)&R32_, n_ * sizeof(float2)));
     NH_TRY(alloc((void **)&Y32_, n_ * sizeof(float2)));
-    if (v32_) NH_TRY(alloc((void **)&Rm132_, n_ * sizeof(float2)));
+    if (v32_) {
+      NH_TRY(alloc((void **)&U32_, n_ * sizeof(float2)));
+      NH_TRY(alloc((void **)&UN32_, n_ * sizeof(float2)));
+      int blocks = 0;
+      sk_grid(c->n_cations, c->n_anions, sk_mixed_roles(c), &blocks,
+              sk_split_lanes(c) ? SkSplit::kLanes : 4);
+      NH_CHECK(blocks <= nparts_, NH_ERR_STATE, "Lanczos: partial sums do not fit");
+    }
   }
   NH_TRY(upload_start(R32_));
 
