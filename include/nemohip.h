@@ -116,8 +116,9 @@ enum { NH_SET_SK_LANES = 5 };
 /* NH_SET_LZ_VECTORS: storage of the Lanczos vectors r_k of a Lanczos run
  * with FP32 SK or matrix-free products: 0 FP64 (with an FP32 copy for the
  * products' gathers), 1 FP32: the step runs in Paige's order, u_k = H q_k
- * - beta_k q_{k-1} and r_{k+1} = u_k - (alpha_k / beta_k) r_k each
- * computed in FP64 and rounded once to FP32, and the product of step k
+ * - beta_k q_{k-1} computed in FP64 and rounded once to FP32, r_{k+1} =
+ * u_k - (alpha_k / beta_k) r_k as two FP32 fmas with the coefficient split
+ * into two FP32 parts, and the product of step k
  * forms r_k from u_{k-1} and r_{k-1} as it reads them, so a step is one
  * launch (no FP64 vector in either pass; alpha, beta and the sums stay
  * FP64); -1 the default (1 where the products use the

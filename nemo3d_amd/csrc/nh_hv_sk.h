@@ -340,23 +340,32 @@ struct SkQuadStage {
   uint4 buf[kSkThreads * kStride];
 };
 
-// FP32 Lanczos vectors: r_k = u_{k-1} - alpha_{k-1} q_{k-1} with q_{k-1} =
-// r_{k-1} / beta_{k-1}, one FP64 fma with a = alpha_{k-1} / beta_{k-1}
-// (lz_defer_coef), rounded once to FP32.  Every kernel that forms r_k (the
-// products that read u_{k-1} and r_{k-1}, the eigenvector pass that writes
-// r_k) evaluates these expressions on the same operands, so they agree bit
-// for bit.  (Measured on a 786,520-atom dot: the same step as an FP32 fma
-// with a rounded to FP32 puts an error along q_{k-1} into every step; the
-// run needed 10,251 steps instead of 7,901, and with another start missed
-// a level.)
-__device__ __forceinline__ double lz_defer_coef(double al, double inv) { return al * inv; }
-__device__ __forceinline__ float lz_defer1(float u, float rp, double a) {
-  return (float)__builtin_fma(-a, (double)rp, (double)u);
+// FP32 Lanczos vectors: r_k = u_{k-1} - a r_{k-1}, a = alpha_{k-1} /
+// beta_{k-1} (so that a r_{k-1} = alpha_{k-1} q_{k-1}), in FP32 with a
+// split into two FP32 parts (a = hi + lo to 2^-48 relative): two fmas, each
+// rounding entry by entry.  The coefficient must not be rounded to FP32
+// alone: its error is the same for every entry, i.e. an error along
+// q_{k-1} in every step, which cost a 786,520-atom dot 10,251 steps
+// instead of 7,901 (and a level, from another start); with the split it
+// takes the same 7,826 steps as one FP64 fma per entry, 6% faster in the
+// product.  Every kernel that forms r_k (the products that read u_{k-1}
+// and r_{k-1}, the eigenvector pass that writes r_k) evaluates these
+// expressions on the same operands, so they agree bit for bit.
+struct LzCoef {
+  float hi, lo;
+};
+__device__ __forceinline__ LzCoef lz_defer_coef(double al, double inv) {
+  const double a = al * inv;
+  const float hi = (float)a;
+  return LzCoef{hi, (float)(a - (double)hi)};
+}
+__device__ __forceinline__ float lz_defer1(float u, float rp, LzCoef a) {
+  return __builtin_fmaf(-a.hi, rp, __builtin_fmaf(-a.lo, rp, u));
 }
 // where the products read r_k from: x holds u_{k-1}, rp r_{k-1}
 struct LzDeferSrc {
   const float2 *rp;
-  double a;  // lz_defer_coef(alpha_{k-1}, 1 / beta_{k-1})
+  LzCoef a;  // lz_defer_coef(alpha_{k-1}, 1 / beta_{k-1})
 };
 __device__ __forceinline__ uint4 lz_defer_chunk(uint4 u, uint4 rp, const LzDeferSrc &d) {
   uint4 o;
@@ -378,7 +387,7 @@ __device__ __forceinline__ void sk_quad_gather(int j, int slot,
                                                R (&xr)[NSP][10],
                                                R (&xi)[NSP][10],
                                                int stride = NSP * 10, int off = 0,
-                                               LzDeferSrc d = {nullptr, 0.0}) {
+                                               LzDeferSrc d = {nullptr, {0.0f, 0.0f}}) {
   static_assert(!DEFER || sizeof(XT) == 8, "deferred r_k needs FP32 vectors");
   using S = SkQuadStage<NSP, XT>;
   constexpr int NB = NSP * 10;
@@ -511,7 +520,7 @@ __device__ __forceinline__ void sk_bonds(int atom, int slot, bool active,
                                          double (&own_re)[(NSP * 10 + 3) / 4],
                                          double (&own_im)[(NSP * 10 + 3) / 4],
                                          SkQuadStage<NSP, XT> *L = nullptr,
-                                         LzDeferSrc d = {nullptr, 0.0}) {
+                                         LzDeferSrc d = {nullptr, {0.0f, 0.0f}}) {
   static_assert(!DEFER || QUAD, "deferred r_k needs the LDS gather");
   R br[NSP][10], bi[NSP][10];
 #pragma unroll
@@ -574,7 +583,7 @@ __device__ __forceinline__ void sk_bonds_split(int atom, int slot, int spin,
                                                double (&own_re)[SkSplit::kOwn],
                                                double (&own_im)[SkSplit::kOwn],
                                                SkQuadStage<1, XT> &L,
-                                               LzDeferSrc d = {nullptr, 0.0}) {
+                                               LzDeferSrc d = {nullptr, {0.0f, 0.0f}}) {
   R br[1][10], bi[1][10];
 #pragma unroll
   for (int c = 0; c < 10; ++c) br[0][c] = bi[0][c] = R(0);

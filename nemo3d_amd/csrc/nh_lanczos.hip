@@ -10,6 +10,9 @@
 // it is one kernel and one reduction: the product forms r_k from u_{k-1}
 // and r_{k-1} as it reads them, computes u_k = H q_k - beta_k q_{k-1} and
 // the partials of alpha_k, |u_k|^2 and |q_k|^2 (k_lz_hv_sk, Paige's order).
+// The reference's REDUCE_OPT variant of the iteration (lanczos.c:441-456)
+// defers the vector update into the next iteration in the same spirit, to
+// merge its two reductions into one.
 // alpha and beta stay in device arrays; the host copies the tridiagonal
 // every ConvCheckSkipRate iterations and analyses it (Sturm bisection in
 // the Erange windows, convergence by the trailing components of the
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
   // DEFER: r_k = lz_defer1(u_{k-1}, r_{k-1}, alpha_{k-1} / beta_{k-1});
   // FUSE: r_{k+1} = lz_defer1(u_k, r_k, alpha_k / beta_k)
   const double alp = DEFER && alpha_prev ? alpha_prev[0] : 0.0;
-  const double ap = lz_defer_coef(alp, invp), af = lz_defer_coef(al, inv);
+  const LzCoef ap = lz_defer_coef(alp, invp), af = lz_defer_coef(al, inv);
   LzDeferSrc dsrc{nullptr, ap};
   if constexpr (DEFER) dsrc.rp = Rp;
   SkQuadStage<NSP, XT> *L = nullptr;
@@ -385,7 +388,7 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 5 : 3) void k_lz_hv_sk
   if constexpr (sizeof(R) == 4) xg = Rv32;
   else xg = Rv;
   const double alp = DEFER && alpha_prev ? alpha_prev[0] : 0.0;
-  const double ap = lz_defer_coef(alp, invp), af = lz_defer_coef(al, inv);
+  const LzCoef ap = lz_defer_coef(alp, invp), af = lz_defer_coef(al, inv);
   LzDeferSrc dsrc{nullptr, ap};
   if constexpr (DEFER) dsrc.rp = Rp;
   __shared__ SkQuadStage<1, XT> stage;

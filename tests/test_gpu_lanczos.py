@@ -244,10 +244,11 @@ def test_mixed_precision_matches_dense(vectors):
     assert H.precision == native.PREC_FP64
 
 
-@pytest.mark.parametrize("layout,vectors,lanes", [("sk", "fp64", -1), ("sk", "fp32", 4),
-                                                   ("sk", "fp32", 8), ("mf", "fp32", 4),
-                                                   ("mf", "fp32", 8)])
-def test_fp32_lanczos_vectors_at_fp32_level(layout, vectors, lanes):
+@pytest.mark.parametrize("model,layout,vectors,lanes", [
+    ("gaas20", "sk", "fp64", -1), ("gaas20", "sk", "fp32", 4), ("gaas20", "sk", "fp32", 8),
+    ("gaas20", "mf", "fp32", 4), ("gaas20", "mf", "fp32", 8), ("si10", "sk", "fp32", -1),
+    ("si10", "mf", "fp32", -1)])
+def test_fp32_lanczos_vectors_at_fp32_level(model, layout, vectors, lanes):
     """Lanczos with FP32 products: the eigenvector pass accumulates the FP32
     copies of r (its ring) with 1/beta folded into the coefficients; the
     vectors' FP64 residuals and eigenvalues sit at the FP32 rounding of H·v,
@@ -255,7 +256,10 @@ def test_fp32_lanczos_vectors_at_fp32_level(layout, vectors, lanes):
     With the Lanczos vectors themselves stored in FP32 (the FP64 step's
     result rounded once) the eigenvector pass must still retrace the first
     pass exactly, or its Ritz vectors would not be eigenvectors."""
-    M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=24)
+    if model == "gaas20":
+        M = _models.gaas_box(cells=(2, 2, 2), strain_nm=0.01, seed=24)
+    else:  # 10-band spinless (one spin block per atom)
+        M = _models.si_box(cells=(3, 3, 3), strain_nm=0.01, seed=25)
     ref = oracle.assemble(M.arrays())
     wd, vd = np.linalg.eigh(ref.dense())
     w = distinct(wd)
@@ -281,7 +285,7 @@ def test_fp32_lanczos_vectors_at_fp32_level(layout, vectors, lanes):
             v = v / np.linalg.norm(v)
             assert np.abs(wd - e).min() < bar_e, (prec, e)
             res = np.linalg.norm(ref.hv(v) - e * v)
-            print(layout, vectors, lanes, "precision", prec, "E", e, "residual / ||H||",
+            print(model, layout, vectors, lanes, "precision", prec, "E", e, "residual / ||H||",
                   res / 40.0, "iterations", r["iterations"], flush=True)
             assert res < bar_r * 40.0, (prec, e, res)
             # the vector lies in the eigenspace of its level (Kramers pair)
