@@ -342,7 +342,7 @@ struct SkQuadStage {
 
 // FP32 Lanczos vectors: r_k = u_{k-1} - a r_{k-1}, a = alpha_{k-1} /
 // beta_{k-1} (so that a r_{k-1} = alpha_{k-1} q_{k-1}), in FP32 with a
-// split into two FP32 parts (a = hi + lo to 2^-48 relative): two fmas, each
+// split into two FP32 parts (a = hi + lo to 2^-48 relative): two fmas,
 // rounding entry by entry.  The coefficient must not be rounded to FP32
 // alone: its error is the same for every entry, i.e. an error along
 // q_{k-1} in every step, which cost a 786,520-atom dot 10,251 steps
@@ -360,7 +360,10 @@ __device__ __forceinline__ LzCoef lz_defer_coef(double al, double inv) {
   return LzCoef{hi, (float)(a - (double)hi)};
 }
 __device__ __forceinline__ float lz_defer1(float u, float rp, LzCoef a) {
-  return __builtin_fmaf(-a.hi, rp, __builtin_fmaf(-a.lo, rp, u));
+  // the cancellation u - hi rp happens inside the first fma (one rounding
+  // of the exact difference), then the small correction: about one ulp of
+  // r in all (the other order adds half an ulp of u, which can be larger)
+  return __builtin_fmaf(-a.lo, rp, __builtin_fmaf(-a.hi, rp, u));
 }
 // where the products read r_k from: x holds u_{k-1}, rp r_{k-1}
 struct LzDeferSrc {

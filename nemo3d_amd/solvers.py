@@ -285,8 +285,10 @@ def eigensolve_mixed(H, num_vb=4, num_cb=4, erange_vb=(-0.6, 0.2),
                      return_vectors=False, refine_split=True):
     """Mixed-precision Lanczos: the Lanczos run and its eigenvector pass
     with FP32 H·v (FP32 values and products inside each block, FP64 sums
-    over the bonds) while the recurrence, dots and the tridiagonal stay
-    FP64; then FP64 residual refinement (refine_eigenpairs) of its Ritz
+    over the bonds) and, with the SK and matrix-free layouts, FP32 Lanczos
+    vectors (nh_context_set NH_SET_LZ_VECTORS, on by default), while the
+    step's arithmetic, the dots and the tridiagonal stay FP64; then FP64
+    residual refinement (refine_eigenpairs) of its Ritz
     pairs to ||H x - e x|| <= tol * hnorm.  return_vectors: True gives the
     eigenvectors on the host, "device" the native.DeviceArray that holds
     them (rows in the order of `order` in the result).  tol32: convergence tolerance of

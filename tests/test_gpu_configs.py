@@ -33,7 +33,11 @@ pytestmark = pytest.mark.gpu
 HNORM = 40.0
 WEIGHTS = os.path.join(os.path.dirname(__file__), "..", "nemo3d_amd", "data",
                        "warmstart.npz")
-DOT_WINDOWS = dict(erange_vb=(0.64, 0.75), erange_cb=(1.47, 1.54))
+# the bench's windows: each holds exactly the 4 wanted levels.  With a wider
+# valence window the reference's rule (done once the window holds NumEigReq
+# converged values) can stop on the 5th valence level, 0.7092 eV, 0.9 meV
+# below the 4th, when that one converges first (seen with FP32 products).
+DOT_WINDOWS = dict(erange_vb=(0.7096, 0.7300), erange_cb=(1.4900, 1.5200))
 PYRAMID_WINDOWS = dict(erange_vb=(0.50, 0.56), erange_cb=(1.43, 1.48))
 
 
