@@ -538,7 +538,8 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_scalars3(
     int nparts, const double *__restrict__ part, double *__restrict__ alpha,
     double *__restrict__ beta) {
   // four independent partial sums per component and thread (the loads of
-  // a trip in flight together), combined in a fixed order
+  // a trip in flight together; constant indices keep them in registers),
+  // combined in a fixed order
   double p[4][3] = {};
   int k = threadIdx.x;
   for (; k + 3 * kRedThreads < nparts; k += 4 * kRedThreads) {
@@ -547,9 +548,11 @@ __global__ __launch_bounds__(kRedThreads) void k_lz_scalars3(
 #pragma unroll
       for (int c = 0; c < 3; ++c) p[u][c] += part[3 * (k + u * kRedThreads) + c];
   }
-  for (int u = 0; k < nparts; k += kRedThreads, ++u)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[u & 3][c] += part[3 * k + c];
+  for (int u = 0; u < 3; ++u)
+    if (k + u * kRedThreads < nparts)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) p[u][c] += part[3 * (k + u * kRedThreads) + c];
   double s[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) s[c] = (p[0][c] + p[1][c]) + (p[2][c] + p[3][c]);
