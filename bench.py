@@ -89,10 +89,11 @@ WINDOWS = WORKLOADS["dot786k"]["windows"]
 def lanczos_memory(n, free_bytes=288e9, k=8):
     """Device bytes of the Lanczos vectors at N = n (SK layout): the
     eigenvalue pass keeps r_k, r_(k-1) and y in FP64 (+ two FP32 copies of r
-    with FP32 products); the eigenvector pass adds the ring of r vectors
-    (2 K FP64 slots, or 2 K FP32 copies with FP32 products; K = 128 steps
-    per flush, halved down to 8 until the ring takes at most half the free
-    memory) and the k FP64 eigenvectors."""
+    with FP32 products; with FP32 Lanczos vectors r and u of two steps in
+    FP32 and one FP64 work vector); the eigenvector pass adds the ring of r
+    vectors (2 K FP64 slots, or 2 K FP32 slots with FP32 products; K = 128
+    steps per flush, halved down to 8 until the ring takes at most half the
+    free memory) and the k FP64 eigenvectors."""
     v64, v32 = n * 16, n * 8
 
     def ring(per_step):  # bytes per step of K: two slots
@@ -108,7 +109,9 @@ def lanczos_memory(n, free_bytes=288e9, k=8):
             "eigenvector_pass_fp64_GB": (3 * v64 + r64 + k * v64) / 1e9,
             "eigenvector_pass_fp64_ring_steps": k64,
             "eigenvector_pass_fp32_products_GB": (3 * v64 + 2 * v32 + r32 + k * v64) / 1e9,
-            "eigenvector_pass_fp32_ring_steps": k32}
+            "eigenvector_pass_fp32_ring_steps": k32,
+            "eigenvalue_pass_fp32_vectors_GB": (4 * v32 + v64) / 1e9,
+            "eigenvector_pass_fp32_vectors_GB": (4 * v32 + v64 + r32 + k * v64) / 1e9}
 
 
 def sk_lanczos_bytes(M, rec_bytes=8, fp32_copy=False, fp32_vectors=False):
