@@ -356,7 +356,7 @@ def main():
     wl = WORKLOADS[args.workload]
     hnorm, windows = wl["hnorm"], wl["windows"]
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); "
+        print("bench.py: one GPU only (no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
