@@ -371,3 +371,20 @@ def test_hv_reproduces_tracemin_example(layout, precision):
         yr = Hc @ x
         err = np.linalg.norm(H.hv(x) - yr) / np.linalg.norm(yr)
         assert err <= tol, err
+
+
+def test_context_settings_are_checked():
+    """nh_context_set refuses values outside each setting's range and
+    unknown settings with NH_ERR_ARG, and leaves the context usable."""
+    M = make("ingaas20_alloy_phi")
+    H = Hamiltonian(M, layout=native.LAYOUT_SK)
+    for what, bad in ((native.SET_LZ_VECTORS, 2), (native.SET_SK_LANES, 5),
+                      (native.SET_SK_ROLES, 3), (native.SET_RING_STEPS, 7),
+                      (native.SET_BLOCK_HV, 3), (native.SET_HV_STAGED, 2), (99, 0)):
+        with pytest.raises(native.NHError):
+            H.set(what, bad)
+    for what in (native.SET_LZ_VECTORS, native.SET_SK_LANES, native.SET_SK_ROLES):
+        H.set(what, -1)
+    X = _models.random_vectors(H.n, 1, seed=5)[0]
+    ref = oracle.assemble(M.arrays())
+    assert np.linalg.norm(H.hv(X, 0.0) - ref.hv(X)) <= 1e-12 * np.linalg.norm(ref.hv(X))
