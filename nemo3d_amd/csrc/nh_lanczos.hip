@@ -181,6 +181,15 @@ struct LzPart3 {
   }
 };
 
+// Epilogue staging of the FP32-vector kernels: the atom's own entries of
+// r_k and r_{k-1} (FP32) and its onsite record
+template <int NSP, int ATOMS = kSkAtomsPerBlock>
+struct SkOwnStage32 {
+  float2 x[ATOMS][NSP * 10];
+  float2 rp[ATOMS][NSP * 10];
+  double2 os[ATOMS][kOnsiteStride / 2];
+};
+
 // XO: storage of the Lanczos vectors r.  double2: FP64 r (own entries,
 // r_prev) with an FP32 copy for the gathers of FP32 products.  float2
 // (FP32 products, LDS gather): the vectors are FP32 and the step runs in
@@ -246,7 +255,95 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
                                              own_im, L, dsrc);
   double local = 0.0;
   LzPart3 p3;
-  if constexpr (QUAD) {
+  if constexpr (V32) {
+    // FP32 vectors: the atom's own entries of r_k (DEFER: of u_{k-1}) and
+    // r_{k-1} are requested in 16-byte pieces of two entries (lane `slot`
+    // takes pieces slot, slot + 4, ...), with the onsite record, before the
+    // epilogue's barriers, and staged in FP32 in the freed LDS; each lane
+    // then finishes its rows 4 i + slot from there
+    static_assert(sizeof(SkOwnStage32<NSP>) <= sizeof(SkQuadStage<NSP, XT>),
+                  "epilogue staging must fit the gather staging");
+    auto &S = *reinterpret_cast<SkOwnStage32<NSP> *>(L->buf);
+    const int la = threadIdx.x / 4;
+    constexpr int NC = NB / 2, NCL = (NC + 3) / 4;
+    constexpr int KOS = SkOwnRegs<NSP>::kOs;
+    float4 cx[NCL], cp[NCL];
+    double2 co[KOS];
+#pragma unroll
+    for (int i = 0; i < NCL; ++i) cx[i] = cp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < KOS; ++i) co[i] = make_double2(0.0, 0.0);
+    if (active) {
+      const float4 *xs = reinterpret_cast<const float4 *>(Rv + (int64_t)atom * NB);
+      const float4 *ps = reinterpret_cast<const float4 *>(Rp + (int64_t)atom * NB);
+#pragma unroll
+      for (int i = 0; i < NCL; ++i) {
+        const int c = 4 * i + slot;
+        if (c < NC) {
+          cx[i] = xs[c];
+          cp[i] = ps[c];
+        }
+      }
+      const double2 *o2 = reinterpret_cast<const double2 *>(onsite + (size_t)atom * kOnsiteStride);
+#pragma unroll
+      for (int i = 0; i < KOS; ++i) {
+        const int c = 4 * i + slot;
+        if (c < kOnsiteStride / 2) co[i] = o2[c];
+      }
+      if constexpr (DEFER) {
+        // r_k from u_{k-1} and r_{k-1}, written out for the next step
+#pragma unroll
+        for (int i = 0; i < NCL; ++i) {
+          const int c = 4 * i + slot;
+          if (c < NC) {
+            cx[i] = make_float4(lz_defer1(cx[i].x, cp[i].x, ap), lz_defer1(cx[i].y, cp[i].y, ap),
+                                lz_defer1(cx[i].z, cp[i].z, ap), lz_defer1(cx[i].w, cp[i].w, ap));
+            if (Rout) reinterpret_cast<float4 *>(Rout + (int64_t)atom * NB)[c] = cx[i];
+          }
+        }
+      }
+    }
+    __syncthreads();  // every lane has read its staged neighbour entries
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < NCL; ++i) {
+        const int c = 4 * i + slot;
+        if (c < NC) {
+          S.x[la][2 * c] = make_float2(cx[i].x, cx[i].y);
+          S.x[la][2 * c + 1] = make_float2(cx[i].z, cx[i].w);
+          S.rp[la][2 * c] = make_float2(cp[i].x, cp[i].y);
+          S.rp[la][2 * c + 1] = make_float2(cp[i].z, cp[i].w);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < KOS; ++i) {
+        const int c = 4 * i + slot;
+        if (c < kOnsiteStride / 2) S.os[la][c] = co[i];
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < NO; ++i) {
+        const int r = 4 * i + slot;
+        if (r >= NB) break;
+        double re = own_re[i], im = own_im[i];
+        onsite_row<NSP>(reinterpret_cast<const double *>(S.os[la]), S.x[la],
+                        r / 10, r % 10, 0.0, re, im);
+        const float2 xf = S.x[la][r];
+        const double2 q = make_double2((double)xf.x * inv, (double)xf.y * inv);
+        const int64_t e = (int64_t)atom * NB + r;
+        const double2 y = make_double2(re * inv, im * inv);
+        const float2 u = lz_u32(y, widen2(S.rp[la][r]), be, invp);
+        if constexpr (FUSE) {
+          Y32[e] = make_float2(lz_defer1(u.x, xf.x, af), lz_defer1(u.y, xf.y, af));
+        } else {
+          Y32[e] = u;
+          p3.add(q, u);
+        }
+      }
+    }
+  } else if constexpr (QUAD) {
     static_assert(sizeof(SkOwnStage<NSP>) <= sizeof(SkQuadStage<NSP, XT>),
                   "epilogue staging must fit the gather staging");
     auto &S = *reinterpret_cast<SkOwnStage<NSP> *>(L->buf);
@@ -261,26 +358,11 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
     double2 rpv[NO];
 #pragma unroll
     for (int i = 0; i < NO; ++i) rpv[i] = make_double2(0.0, 0.0);
-    if constexpr (FUSE || DEFER) {
+    if constexpr (FUSE) {
       if (active && Rp) {
 #pragma unroll
         for (int i = 0; i < NO; ++i)
           if (4 * i + slot < NB) rpv[i] = widen2(Rp[(int64_t)atom * NB + 4 * i + slot]);
-      }
-    }
-    if constexpr (DEFER) {
-      // own entries of r_k (the lane's rows 4 i + slot, as loaded) from
-      // u_{k-1} and r_{k-1}; written out for the next step
-      if (active) {
-#pragma unroll
-        for (int i = 0; i < NO; ++i) {
-          const int r = 4 * i + slot;
-          if (r >= NB) break;
-          const float2 rk = make_float2(lz_defer1((float)own.x[i].x, (float)rpv[i].x, ap),
-                                        lz_defer1((float)own.x[i].y, (float)rpv[i].y, ap));
-          own.x[i] = widen2(rk);
-          if (Rout) Rout[(int64_t)atom * NB + r] = rk;
-        }
       }
     }
     __syncthreads();  // every lane has read its staged neighbour entries
@@ -297,24 +379,13 @@ __global__ __launch_bounds__(kSkThreads, sizeof(R) == 4 ? 4 : 2) void k_lz_hv_sk
         const double2 q = make_double2(own.x[i].x * inv, own.x[i].y * inv);
         const int64_t e = (int64_t)atom * NB + r;
         double2 y = make_double2(re * inv, im * inv);
-        if constexpr (V32) {
-          const float2 u = lz_u32(y, rpv[i], be, invp);
-          if constexpr (FUSE) {
-            Y32[e] = make_float2(lz_defer1(u.x, (float)own.x[i].x, af),
-                                 lz_defer1(u.y, (float)own.x[i].y, af));
-          } else {
-            Y32[e] = u;
-            p3.add(q, u);
-          }
-        } else {
-          if constexpr (FUSE) {
-            y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
-            if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
-          }
-          if (Q) Q[e] = q;
-          Y[e] = y;
-          local += q.x * y.x + q.y * y.y;
+        if constexpr (FUSE) {
+          y = lz_three_term_v(y, q, Rp != nullptr, rpv[i], al, be, invp);
+          if (Y32) Y32[e] = make_float2((float)y.x, (float)y.y);
         }
+        if (Q) Q[e] = q;
+        Y[e] = y;
+        local += q.x * y.x + q.y * y.y;
       }
     }
   } else if (active) {
