@@ -345,6 +345,8 @@ def main():
                     help="Hamiltonian layout of the solves: stored SK records "
                          "or matrix-free (records recomputed per product)")
     ap.add_argument("--no-layout-probe", action="store_true")
+    ap.add_argument("--no-fp64-vectors-run", action="store_true",
+                    help="skip the untimed mixed solve with FP64 Lanczos vectors")
     ap.add_argument("--lz-vectors", choices=["fp64", "fp32"], default="fp32",
                     help="storage of the Lanczos vectors of the FP32-product stage "
                          "(the three-term step is FP64 either way)")
@@ -521,7 +523,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 products / c128 recurrences" if mixed is not None else "c128",
+        "dtype": ("c64 products and Lanczos vectors / c128 step arithmetic, dots, "
+                  "tridiagonal, refinement" if (mixed is not None and v32) else
+                  "c64 products / c128 recurrences" if mixed is not None else "c128"),
         "data": "synthetic",
         "config": cfg,
         "roofline": rooflines[0] if rooflines else None,
@@ -542,6 +546,22 @@ def main():
         cfg["lanczos_vector_memory"] = lanczos_memory(M.n)
     cfg["layout"] = {"sk": "stored SK records", "mf": "matrix-free"}[args.layout]
     cfg["lanczos_vectors_fp32_stage"] = args.lz_vectors
+    if mixed is not None and v32 and not args.no_fp64_vectors_run:
+        # the same mixed solve with FP64 Lanczos vectors (FP32 products only,
+        # the recurrence's vectors FP64 as in the reference), once, outside
+        # the timed region
+        H.set(native.SET_LZ_VECTORS, 0)
+        t64, m64 = run_mixed()
+        H.set(native.SET_LZ_VECTORS, 1)
+        d64 = (np.sort(mixed["eigenvalues"]) - np.sort(m64["eigenvalues"])
+               if len(m64["eigenvalues"]) == len(mixed["eigenvalues"]) else None)
+        cfg["mixed_fp64_vectors"] = {
+            "s": t64, "fp32_stage_iterations": int(m64["stage32"]["iterations"]),
+            "converged": bool(m64["converged"]),
+            "eig_diff_vs_fp32_vectors_eV": None if d64 is None else d64.tolist(),
+            "speedup_vs_plain": (cfg["plain_fp64_s"] / t64) if plain is not None else None}
+        log("mixed with FP64 Lanczos vectors: %.2f s, FP32 stage %d iterations" % (
+            t64, m64["stage32"]["iterations"]))
     if args.learned_routes and net is not None:
         cfg["learned_block_routes"] = learned_routes(H, net, inputs, windows, hnorm)
     if not args.no_layout_probe:
