@@ -105,39 +105,55 @@ __global__ __launch_bounds__(128) void k_conv_fwd(
   }
 }
 
-// head 1x1x1: out[p][co] = b[co] + sum_ci W[co][ci] a[p][ci], co in a chunk
-__global__ __launch_bounds__(128) void k_head_fwd(
+// head 1x1x1: out[p][co] = b[co] + sum_ci W[co][ci] a[p][ci].  Lane =
+// output channel of a 64-channel chunk (blockIdx.y), so a wavefront writes
+// 64 consecutive outputs of one point and reads that point's inputs as one
+// broadcast; each wavefront walks groups of kHeadPts points.  The chunk's weights sit
+// in LDS as FP32, [ci][64] (lane-consecutive, no bank conflicts).
+constexpr int kHeadPts = 8;
+__global__ __launch_bounds__(256) void k_head_fwd(
     int64_t P, int ci_n, int co_n, const float *__restrict__ a,
     const uint16_t *__restrict__ W, const uint16_t *__restrict__ b,
     float *__restrict__ out) {
-  extern __shared__ uint16_t wsh[];  // [64][ci_n]
+  extern __shared__ float whs[];  // [ci_n][64]
   const int co0 = blockIdx.y * 64;
-  const int nco = min(64, co_n - co0);
-  for (int t = threadIdx.x; t < nco * ci_n; t += blockDim.x)
-    wsh[t] = W[(size_t)co0 * ci_n + t];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < 64 * ci_n; t += blockDim.x) {
+    const int ci = t >> 6, c = t & 63;
+    whs[t] = co0 + c < co_n ? bf16_to_f(W[(size_t)(co0 + c) * ci_n + ci]) : 0.0f;
+  }
   __syncthreads();
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  float av[64];
-  for (int ci = 0; ci < ci_n && ci < 64; ++ci) av[ci] = a[p * ci_n + ci];
-  for (int c = 0; c < nco; ++c) {
-    float s = bf16_to_f(b[co0 + c]);
-    for (int ci = 0; ci < ci_n; ++ci) s += bf16_to_f(wsh[c * ci_n + ci]) * av[ci];
-    out[p * co_n + co0 + c] = s;
+  const int co = co0 + lane;
+  const bool ok = co < co_n;
+  const float bias = ok ? bf16_to_f(b[co]) : 0.0f;
+  // groups of kHeadPts points, grid-stride (the grid is capped)
+  for (int64_t g = (int64_t)blockIdx.x * 4 + w; g * kHeadPts < P;
+       g += (int64_t)gridDim.x * 4) {
+    for (int k = 0; k < kHeadPts; ++k) {
+      const int64_t p = g * kHeadPts + k;
+      if (p >= P) break;
+      const float *ap = a + p * ci_n;
+      float s = bias;
+      for (int ci = 0; ci < ci_n; ++ci) s += whs[ci * 64 + lane] * ap[ci];
+      if (ok) out[p * co_n + co] = s;
+    }
   }
 }
 
-// head backward, data: da[p][ci] = sum_co W[co][ci] dout[p][co]
-__global__ __launch_bounds__(128) void k_head_bwd_data(
+// head backward, data: da[p][ci] = sum_co W[co][ci] dout[p][co].  Lane =
+// input channel (ci_n <= 64 lanes of a wavefront per point), so the
+// point's dout is read as broadcasts and da written contiguously; W is
+// read lane-consecutive from the caches; grid-stride over points.
+__global__ __launch_bounds__(256) void k_head_bwd_data(
     int64_t P, int ci_n, int co_n, const float *__restrict__ dout,
     const uint16_t *__restrict__ W, float *__restrict__ da) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  for (int ci = 0; ci < ci_n; ++ci) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane >= ci_n) return;
+  for (int64_t p = (int64_t)blockIdx.x * 4 + w; p < P; p += (int64_t)gridDim.x * 4) {
+    const float *dp = dout + p * co_n;
     float s = 0.0f;
-    for (int co = 0; co < co_n; ++co)
-      s += bf16_to_f(W[(size_t)co * ci_n + ci]) * dout[p * co_n + co];
-    da[p * ci_n + ci] = s;
+    for (int co = 0; co < co_n; ++co) s += bf16_to_f(W[(size_t)co * ci_n + lane]) * dp[co];
+    da[p * ci_n + lane] = s;
   }
 }
 
@@ -347,8 +363,8 @@ nh_status forward(nh_cnn *c, int gx, int gy, int gz) {
     ci = c->width;
   }
   const int L = c->layers;
-  hipLaunchKernelGGL(k_head_fwd, dim3(g1(P, 128), (c->c_out + 63) / 64),
-                     dim3(128), 64 * ci * sizeof(uint16_t), st, P, ci, c->c_out,
+  hipLaunchKernelGGL(k_head_fwd, dim3(g1(P, 4 * kHeadPts), (c->c_out + 63) / 64),
+                     dim3(256), 64 * ci * sizeof(float), st, P, ci, c->c_out,
                      src, c->bf + c->w_off[L], c->bf + c->b_off[L], c->out);
   NH_HIP(hipGetLastError());
   return NH_OK;
@@ -363,7 +379,8 @@ nh_status backward(nh_cnn *c, int gx, int gy, int gz, const float *dout) {
   hipLaunchKernelGGL(k_head_bwd_w, dim3(c->c_out), dim3(256), 0, st, P, W,
                      c->c_out, dout, alast, c->grad + c->w_off[L],
                      c->grad + c->b_off[L]);
-  hipLaunchKernelGGL(k_head_bwd_data, dim3(g1(P, 128)), dim3(128), 0, st, P, W,
+  NH_CHECK(W <= 64, NH_ERR_ARG, "network width above 64");
+  hipLaunchKernelGGL(k_head_bwd_data, dim3(g1(P, 4)), dim3(256), 0, st, P, W,
                      c->c_out, dout, c->bf + c->w_off[L], c->dact);
   for (int l = L - 1; l >= 0; --l) {
     const int ci = l == 0 ? c->c_in : W;
