@@ -3,8 +3,9 @@ the FP64 oracle.
 
 * 786,520-atom InAs dome dot (base 20 nm, height 5 nm, 1-ML wetting layer,
   30 x 30 x 20 nm GaAs box, VFF positions, sp3d5s* + SO, N = 15.7M): plain
-  FP64 Lanczos; mixed precision (FP32 products, FP64 recurrences and
-  refinement) from the learned start; learned against CyclicRandom start.
+  FP64 Lanczos; mixed precision (FP32 products and Lanczos vectors, FP64
+  step arithmetic and refinement) from the learned start, in the stored
+  and the matrix-free layouts; learned against CyclicRandom start.
 * The same geometry with a seeded In0.8Ga0.2As alloy in the dot and
   wetting layer: plain and mixed.
 * 4,336,640-atom truncated-pyramid In0.7Ga0.3As dot in a 50 x 50 x 40 nm
@@ -141,6 +142,26 @@ def test_dot_learned_start_needs_fewer_iterations(dot):
     print("iterations: learned %d, CyclicRandom %d" % (il, ir), flush=True)
     assert il < ir
     assert np.abs(np.sort(rl["eigenvalues"]) - np.sort(rr["eigenvalues"])).max() <= 1e-6
+
+
+@pytest.mark.timeout(300)
+def test_dot_matrix_free_mixed_against_oracle(dot):
+    """The same dot with the matrix-free layout (bond records recomputed
+    from the positions in every product, the reference's
+    HamCompute=Recompute): the mixed solve (FP32 products and Lanczos
+    vectors, 8 lanes per atom) from the learned start against the FP64
+    oracle, and its eigenvalues against the stored-layout plain run."""
+    if dot.plain is None:
+        dot.plain = plain(dot.H, DOT_WINDOWS)
+    Hm = Hamiltonian(dot.M, layout=native.LAYOUT_MATRIX_FREE)
+    try:
+        r = mixed(Hm, DOT_WINDOWS, learned_start(Hm))
+        oracle_check(dot.ref, r["eigenvalues"], r["vectors"], "matrix-free mixed")
+        d = np.sort(r["eigenvalues"]) - np.sort(dot.plain["eigenvalues"])
+        print("matrix-free mixed - plain (eV)", d, flush=True)
+        assert np.abs(d).max() <= 1e-6
+    finally:
+        Hm.close()
 
 
 def tight_windows(eigenvalues, split, pad=2e-5):
