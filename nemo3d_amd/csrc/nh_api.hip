@@ -121,8 +121,8 @@ size_t pool_free_bytes(const nh_context *c) {
 nh_status ensure_buffers(nh_context *c, int64_t elems) {
   if (elems <= c->buf_cap) return NH_OK;
   NH_HIP(hipStreamSynchronize(c->stream));
-  hipFree(c->xbuf);
-  hipFree(c->ybuf);
+  (void)hipFree(c->xbuf);
+  (void)hipFree(c->ybuf);
   c->xbuf = c->ybuf = nullptr;
   c->buf_cap = 0;
   NH_TRY(device_malloc(c, (void **)&c->xbuf, elems * sizeof(nh_cplx)));
@@ -144,18 +144,18 @@ static void destroy(nh_context *c) {
     std::lock_guard<std::mutex> g(g_ctx_mu);
     g_ctxs.erase(std::remove(g_ctxs.begin(), g_ctxs.end(), c), g_ctxs.end());
   }
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   void *ptrs[] = {c->parmat, c->strnmat, c->species, c->is_cation, c->nbr,
                   c->bond_q, c->disp, c->phi, c->cations, c->anions, c->bond_of,
                   c->rev_slot, c->onsite, c->blocks64, c->blocks32,
                   c->onsite32, c->sk64, c->sk32, c->corr, c->red, c->xbuf,
                   c->ybuf};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  hipStreamSynchronize(c->stream);
-  for (auto &b : c->pool) hipFree(b.p);
-  if (c->hred) hipHostFree(c->hred);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &b : c->pool) (void)hipFree(b.p);
+  if (c->hred) (void)hipHostFree(c->hred);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 

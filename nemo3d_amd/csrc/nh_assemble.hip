@@ -254,16 +254,16 @@ inline int grid_for(int64_t n, int block) {
 }  // namespace
 
 static void free_ham(nh_context *c) {
-  hipFree(c->onsite);
-  hipFree(c->blocks64);
-  hipFree(c->blocks32);
-  hipFree(c->onsite32);
-  hipFree(c->sk64);
-  hipFree(c->sk32);
-  hipFree(c->corr);
-  hipFree(c->mf_geo);
-  hipFree(c->mf_q);
-  hipFree(c->mf_pair);
+  (void)hipFree(c->onsite);
+  (void)hipFree(c->blocks64);
+  (void)hipFree(c->blocks32);
+  (void)hipFree(c->onsite32);
+  (void)hipFree(c->sk64);
+  (void)hipFree(c->sk32);
+  (void)hipFree(c->corr);
+  (void)hipFree(c->mf_geo);
+  (void)hipFree(c->mf_q);
+  (void)hipFree(c->mf_pair);
   c->mf_geo = nullptr;
   c->mf_q = nullptr;
   c->mf_pair = nullptr;
@@ -326,7 +326,7 @@ nh_status assemble(nh_context *c, int layout, int precision) {
                      c->corr, c->onsite);
   NH_HIP(hipGetLastError());
   NH_HIP(hipStreamSynchronize(c->stream));
-  hipFree(c->corr);  // only the assembly needs the per-bond terms
+  (void)hipFree(c->corr);  // only the assembly needs the per-bond terms
   c->corr = nullptr;
   c->layout = layout;
   c->precision = NH_PREC_FP64;

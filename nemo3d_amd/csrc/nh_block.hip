@@ -47,7 +47,7 @@ class BlockLanczos {
  public:
   explicit BlockLanczos(nh_context *c) : c_(c) {}
   ~BlockLanczos() {
-    for (void *q : mem_) hipFree(q);
+    for (void *q : mem_) (void)hipFree(q);
   }
   nh_status run(const nh_blanczos_opts *o, nh_eig_result *res);
 

@@ -327,12 +327,12 @@ int g1(int64_t n, int b = 256) {
 
 nh_status ensure_acts(nh_cnn *c, int64_t P) {
   if (P <= c->cap_points) return NH_OK;
-  for (float *q : c->pre) hipFree(q);
-  for (float *q : c->act) hipFree(q);
-  hipFree(c->in);
-  hipFree(c->out);
-  hipFree(c->dact);
-  hipFree(c->dtmp);
+  for (float *q : c->pre) (void)hipFree(q);
+  for (float *q : c->act) (void)hipFree(q);
+  (void)hipFree(c->in);
+  (void)hipFree(c->out);
+  (void)hipFree(c->dact);
+  (void)hipFree(c->dtmp);
   c->pre.assign(c->layers, nullptr);
   c->act.assign(c->layers, nullptr);
   for (int l = 0; l < c->layers; ++l) {
@@ -489,10 +489,15 @@ nh_status nh_cnn_create(nh_context *ctx, int c_in, int width, int layers,
   mk((void **)&c->m1, c->n_params * sizeof(float));
   mk((void **)&c->m2, c->n_params * sizeof(float));
   if (st == NH_OK) {
-    hipMemcpy(c->master, w.data(), c->n_params * sizeof(float), hipMemcpyHostToDevice);
-    hipMemset(c->m1, 0, c->n_params * sizeof(float));
-    hipMemset(c->m2, 0, c->n_params * sizeof(float));
-    hipMemset(c->grad, 0, c->n_params * sizeof(float));
+    hipError_t e = hipMemcpy(c->master, w.data(), c->n_params * sizeof(float),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c->m1, 0, c->n_params * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(c->m2, 0, c->n_params * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(c->grad, 0, c->n_params * sizeof(float));
+    if (e != hipSuccess) {
+      nh_cnn_destroy(c);
+      NH_CHECK(false, NH_ERR_HIP, "network set-up: %s", hipGetErrorString(e));
+    }
     st = sync_bf16(c);
   }
   if (st != NH_OK) {
@@ -505,13 +510,13 @@ nh_status nh_cnn_create(nh_context *ctx, int c_in, int width, int layers,
 
 void nh_cnn_destroy(nh_cnn *c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  for (float *q : c->pre) hipFree(q);
-  for (float *q : c->act) hipFree(q);
+  (void)hipSetDevice(c->device);
+  for (float *q : c->pre) (void)hipFree(q);
+  for (float *q : c->act) (void)hipFree(q);
   void *ps[] = {c->master, c->bf, c->grad, c->m1, c->m2, c->in, c->out,
                 c->dact, c->dtmp, c->part};
   for (void *q : ps)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   delete c;
 }
 
@@ -587,14 +592,21 @@ nh_status nh_cnn_start_block(nh_cnn *c, const float *feat, int gx, int gy,
   int32_t *dp = nullptr, *ds = nullptr;
   NH_TRY(device_malloc(x, (void **)&dp, A * sizeof(int32_t)));
   NH_TRY(device_malloc(x, (void **)&ds, A * sizeof(int32_t)));
-  hipMemcpy(dp, atom_point, A * sizeof(int32_t), hipMemcpyHostToDevice);
-  hipMemcpy(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice);
+  {
+    hipError_t e = hipMemcpy(dp, atom_point, A * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(dp);
+      (void)hipFree(ds);
+      NH_CHECK(false, NH_ERR_HIP, "nh_cnn_start_block: %s", hipGetErrorString(e));
+    }
+  }
   double2 *Y = reinterpret_cast<double2 *>(block);
   double2 *tmp = nullptr;
   if (ptr_kind == NH_PTR_HOST) {
     if (device_malloc(x, (void **)&tmp, (size_t)n_states * x->n * sizeof(double2)) != NH_OK) {
-      hipFree(dp);
-      hipFree(ds);
+      (void)hipFree(dp);
+      (void)hipFree(ds);
       NH_CHECK(false, NH_ERR_NOMEM, "nh_cnn_start_block: out of memory");
     }
     Y = tmp;
@@ -606,9 +618,9 @@ nh_status nh_cnn_start_block(nh_cnn *c, const float *feat, int gx, int gy,
     e = hipMemcpyAsync(block, tmp, (size_t)n_states * x->n * sizeof(double2),
                        hipMemcpyDeviceToHost, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
-  hipFree(dp);
-  hipFree(ds);
-  if (tmp) hipFree(tmp);
+  (void)hipFree(dp);
+  (void)hipFree(ds);
+  if (tmp) (void)hipFree(tmp);
   NH_CHECK(e == hipSuccess, NH_ERR_HIP, "nh_cnn_start_block: %s", hipGetErrorString(e));
   return NH_OK;
 }
@@ -639,7 +651,7 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
     return NH_OK;
   };
   auto cleanup = [&]() {
-    for (void *q : mem) hipFree(q);
+    for (void *q : mem) (void)hipFree(q);
   };
   int32_t *dp, *ds;
   double2 *Y, *V, *G;
@@ -658,13 +670,22 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
     return st;
   }
   hipStream_t s = x->stream;
-  hipMemcpyAsync(dp, atom_point, A * sizeof(int32_t), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice, s);
-  hipMemsetAsync(Y, 0, (size_t)8 * N * sizeof(double2), s);
-  hipMemsetAsync(V, 0, (size_t)16 * N * sizeof(double2), s);
+  // an asynchronous call that fails ends the step with the buffers released
+#define TS_HIP(call)                                                            \
+  do {                                                                          \
+    const hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                                     \
+      cleanup();                                                                \
+      NH_CHECK(false, NH_ERR_HIP, "train step: %s", hipGetErrorString(e_));    \
+    }                                                                           \
+  } while (0)
+  TS_HIP(hipMemcpyAsync(dp, atom_point, A * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  TS_HIP(hipMemcpyAsync(ds, atom_slot, A * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  TS_HIP(hipMemsetAsync(Y, 0, (size_t)8 * N * sizeof(double2), s));
+  TS_HIP(hipMemsetAsync(V, 0, (size_t)16 * N * sizeof(double2), s));
   // targets may be host or device memory (unified addressing)
-  hipMemcpyAsync(V, targets, (size_t)n_targets * N * sizeof(double2),
-                 hipMemcpyDefault, s);
+  TS_HIP(hipMemcpyAsync(V, targets, (size_t)n_targets * N * sizeof(double2),
+                 hipMemcpyDefault, s));
   hipLaunchKernelGGL(k_map_out, dim3(g1(A * x->nb)), dim3(256), 0, s, A, x->nb,
                      ne, c->c_out, dp, ds, c->out, Y, N);
   // C = V^H Y (16 x 8, two 8x8 halves) and the Gram Y^H Y
@@ -673,14 +694,14 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
     hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, 8,
                        V + (int64_t)half * 8 * N, N, Y, N, part);
     hipLaunchKernelGGL(k_sum_cols, dim3(128), dim3(128), 0, s, 512, 128, part, sum);
-    hipMemcpyAsync(h.data() + half * 128, sum, 128 * sizeof(double),
-                   hipMemcpyDeviceToHost, s);
+    TS_HIP(hipMemcpyAsync(h.data() + half * 128, sum, 128 * sizeof(double),
+                   hipMemcpyDeviceToHost, s));
     // the next k_sum_cols overwrites `sum`: wait for the copy
-    hipStreamSynchronize(s);
+    TS_HIP(hipStreamSynchronize(s));
   }
   hipLaunchKernelGGL((k_gram<8>), dim3(512), dim3(256), 0, s, N, 8, 8, Y, N, Y, N, part);
   hipLaunchKernelGGL(k_sum_cols, dim3(128), dim3(128), 0, s, 512, 128, part, sum);
-  hipMemcpyAsync(hb.data(), sum, 128 * sizeof(double), hipMemcpyDeviceToHost, s);
+  TS_HIP(hipMemcpyAsync(hb.data(), sum, 128 * sizeof(double), hipMemcpyDeviceToHost, s));
   hipError_t e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     cleanup();
@@ -713,25 +734,26 @@ nh_status nh_cnn_train_step(nh_cnn *c, const float *feat, int gx, int gy, int gz
       hm[2 * k] = M[half * 64 + k].real();
       hm[2 * k + 1] = M[half * 64 + k].imag();
     }
-    hipMemcpyAsync(dM, hm.data(), 128 * sizeof(double), hipMemcpyHostToDevice, s);
+    TS_HIP(hipMemcpyAsync(dM, hm.data(), 128 * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_block_gemm<8>), dim3(g1(N)), dim3(256), 0, s, N, 8, 8,
                        V + (int64_t)half * 8 * N, N, G, N, dM, half ? 1.0 : 0.0);
-    hipStreamSynchronize(s);  // hm is rewritten next
+    TS_HIP(hipStreamSynchronize(s));  // hm is rewritten next
   }
   for (int k = 0; k < 64; ++k) {
     hm[2 * k] = D[k].real();
     hm[2 * k + 1] = 0.0;
   }
-  hipMemcpyAsync(dM, hm.data(), 128 * sizeof(double), hipMemcpyHostToDevice, s);
+  TS_HIP(hipMemcpyAsync(dM, hm.data(), 128 * sizeof(double), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL((k_block_gemm<8>), dim3(g1(N)), dim3(256), 0, s, N, 8, 8, Y,
                      N, G, N, dM, 1.0);
   // dL/d(out): real and imaginary parts of the complex gradient
-  hipMemsetAsync(c->dtmp, 0, P * c->c_out * sizeof(float), s);
+  TS_HIP(hipMemsetAsync(c->dtmp, 0, P * c->c_out * sizeof(float), s));
   hipLaunchKernelGGL(k_map_grad, dim3(g1(A * x->nb)), dim3(256), 0, s, A, x->nb,
                      ne, c->c_out, dp, ds, G, N, c->dtmp);
   // backward needs a dout buffer that survives: move it into c->out
-  hipMemcpyAsync(c->out, c->dtmp, P * c->c_out * sizeof(float),
-                 hipMemcpyDeviceToDevice, s);
+  TS_HIP(hipMemcpyAsync(c->out, c->dtmp, P * c->c_out * sizeof(float),
+                 hipMemcpyDeviceToDevice, s));
+#undef TS_HIP
   e = hipStreamSynchronize(s);
   if (e == hipSuccess) {
     nh_status bs = backward(c, gx, gy, gz, c->out);

@@ -183,11 +183,11 @@ struct KernelTimer {
   ~KernelTimer() { release(); }
   void release() {
     for (auto &e : ev) {
-      hipEventDestroy(e.first);
-      hipEventDestroy(e.second);
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
     }
     ev.clear();
-    if (pending) hipEventDestroy(pending);
+    if (pending) (void)hipEventDestroy(pending);
     pending = nullptr;
   }
   nh_status begin(hipStream_t s) {

@@ -865,8 +865,8 @@ class LanczosRun {
   ~LanczosRun() {
     release_events();
     for (void *p : allocs_) pool_release(c_, p);
-    if (pin_) hipHostFree(pin_);
-    if (fetch_ev_) hipEventDestroy(fetch_ev_);
+    if (pin_) (void)hipHostFree(pin_);
+    if (fetch_ev_) (void)hipEventDestroy(fetch_ev_);
   }
   nh_status run(const nh_lanczos_opts *o, nh_eig_result *res);
 
@@ -939,8 +939,8 @@ class LanczosRun {
   int hv_timed(int pass = 0) const { return hv_timed_[pass]; }
   void release_events() {
     for (auto &e : ev_) {
-      hipEventDestroy(e.start);
-      hipEventDestroy(e.stop);
+      (void)hipEventDestroy(e.start);
+      (void)hipEventDestroy(e.stop);
     }
     ev_.clear();
   }

@@ -77,7 +77,7 @@ extern "C" {
 nh_status nh_shim_bind(nh_context *ctx) {
   using nh::g_shim;
   if (g_shim.wf) {
-    hipFree(g_shim.wf);
+    (void)hipFree(g_shim.wf);
     g_shim.wf = nullptr;
   }
   g_shim.energy.clear();
@@ -94,7 +94,7 @@ nh_status nh_shim_set_wavefunctions(int nwf, const double *energy,
   NH_CHECK(nwf >= 0 && (nwf == 0 || (energy && psi)), NH_ERR_ARG,
            "bad wave function table");
   NH_HIP(hipSetDevice(c->device));
-  if (g_shim.wf) hipFree(g_shim.wf);
+  if (g_shim.wf) (void)hipFree(g_shim.wf);
   g_shim.wf = nullptr;
   g_shim.energy.assign(energy, energy + nwf);
   if (nwf == 0) return NH_OK;

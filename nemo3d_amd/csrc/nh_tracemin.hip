@@ -26,7 +26,7 @@ class Tracemin {
  public:
   explicit Tracemin(nh_context *c) : c_(c) {}
   ~Tracemin() {
-    for (void *q : mem_) hipFree(q);
+    for (void *q : mem_) (void)hipFree(q);
   }
   nh_status run(const nh_tracemin_opts *o, nh_eig_result *res);
 

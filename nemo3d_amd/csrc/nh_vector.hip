@@ -39,8 +39,8 @@ double *noalloc_scratch(int device) {
 
 nh_status ensure_reduce(nh_context *c, int blocks) {
   if (blocks <= c->red_cap) return NH_OK;
-  hipFree(c->red);
-  hipHostFree(c->hred);
+  (void)hipFree(c->red);
+  (void)hipHostFree(c->hred);
   c->red = nullptr;
   c->hred = nullptr;
   const int cap = blocks < 4096 ? 4096 : blocks;

@@ -243,7 +243,7 @@ class Vff {
  public:
   Vff(nh_context *c, VffArgs a) : c_(c), g_(a) {}
   ~Vff() {
-    for (void *p : mem_) hipFree(p);
+    for (void *p : mem_) (void)hipFree(p);
   }
   nh_status run(int max_iter, double gtol, double *out_disp, double *info);
 
@@ -378,7 +378,7 @@ namespace {
 // allocation), freed with the object.
 class VffTables {
  public:
-  ~VffTables() { hipFree(mem_); }
+  ~VffTables() { (void)hipFree(mem_); }
   nh_status upload(nh_context *c, const nh_vff_params *p) {
     NH_CHECK(p && p->alpha && p->beta && p->a0 &&
                  (p->model == NH_VFF_KEATING ||
@@ -437,7 +437,7 @@ nh_status energy_grad(nh_context *ctx, const nh_vff_params *p,
       e = hipMemcpy(res, m[2] + 2 * (blocks + 1), 2 * sizeof(double),
                     hipMemcpyDeviceToHost);
   }
-  for (double *q : m) hipFree(q);
+  for (double *q : m) (void)hipFree(q);
   NH_TRY(st);
   NH_CHECK(e == hipSuccess, NH_ERR_HIP, "nh_vff_energy_grad: %s", hipGetErrorString(e));
   *energy = res[0];
