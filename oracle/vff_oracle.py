@@ -31,14 +31,21 @@ NEAR_MAX = 0.001
 
 
 def alpha_eff(al0, A, r2, d2):
+    # absent bonds (d2 = 0) give non-finite values here that the callers mask
     delt = 1.0 - NEAR_MAX
     with np.errstate(divide="ignore", invalid="ignore"):
         inside = r2 < d2 * (1.0 + delt * 2.0 / (3.0 * A))
-    return np.where(inside, al0 * (1.0 - A * (r2 / d2 - 1.0)),
-                    al0 * (1.0 - delt / 1.5))
+        return np.where(inside, al0 * (1.0 - A * (r2 / d2 - 1.0)),
+                        al0 * (1.0 - delt / 1.5))
 
 
 def beta_eff(be0, B, C, d1, d2, r1, r2):
+    # absent legs give non-finite values that the callers mask
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return _beta_eff(be0, B, C, d1, d2, r1, r2)
+
+
+def _beta_eff(be0, B, C, d1, d2, r1, r2):
     d10 = np.sqrt((d1 * d1).sum(-1))
     d20 = np.sqrt((d2 * d2).sum(-1))
     r10 = np.sqrt((r1 * r1).sum(-1))
